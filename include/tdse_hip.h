@@ -1,0 +1,172 @@
+/*
+ * tdse_hip — C ABI of the MI355X (gfx950) mesh-TDSE evolution library.
+ *
+ * Plain pointers and sizes only; every function returns 0 on success and a
+ * negative code on failure, with the message available from
+ * tdse_last_error(ctx) (ctx may be NULL for errors raised before a context
+ * exists). No C++ exception crosses this boundary. complex128 arrays are
+ * interleaved (re, im) doubles.
+ *
+ * Reference interfaces each entry point replaces are cited per declaration
+ * (paths relative to the JoshKarpel/ionization source tree).
+ */
+#ifndef TDSE_HIP_H
+#define TDSE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tdse_ctx tdse_ctx;
+
+enum {
+    TDSE_OK = 0,
+    TDSE_ERR_ARG = -1,     /* invalid argument or layout */
+    TDSE_ERR_HIP = -2,     /* HIP runtime error */
+    TDSE_ERR_NOMEM = -3,   /* device pool exhausted */
+    TDSE_ERR_STATE = -4    /* call not valid in the current state */
+};
+
+/* Library ABI version (increments when a signature changes). */
+int tdse_abi_version(void);
+
+/* Last error message of a context (or of the library when ctx is NULL). */
+const char *tdse_last_error(const tdse_ctx *ctx);
+
+/*
+ * Create / destroy a context on one device: one HIP stream, one device memory
+ * pool of pool_bytes reserved up front (all simulation buffers are carved
+ * from it; nothing is allocated per step). Replaces the per-process state of
+ * a reference run (one MeshSimulation per process, ionization/mesh/sims.py:36-118).
+ */
+int tdse_ctx_create(int device, size_t pool_bytes, tdse_ctx **out);
+int tdse_ctx_destroy(tdse_ctx *ctx);
+int tdse_sync(tdse_ctx *ctx);
+int tdse_pool_usage(tdse_ctx *ctx, size_t *used_bytes, size_t *total_bytes);
+
+/*
+ * Tridiagonal solve x = T^-1 d for one complex128 system of length n >= 1,
+ * sub[i] multiplying x[i-1] (sub[0] ignored) and sup[i] multiplying x[i+1]
+ * (sup[n-1] ignored). Host arrays in, host array out; inputs unchanged.
+ * Replaces cy.tdma(matrix, d) (ionization/cy.pyx:9-50) as called by
+ * TDMAOperator._apply (ionization/mesh/mesh_operators.py:109-111); the
+ * caller passes the dia matrix rows data[0], data[1], data[2] re-aligned as
+ * cy.pyx:19-22 does.
+ */
+int tdse_tdma(tdse_ctx *ctx, const double *sub, const double *diag, const double *sup, const double *d,
+              double *x, int64_t n);
+
+/*
+ * Many independent systems packed back to back: system i occupies
+ * [offsets[i], offsets[i] + lengths[i]) of every array. Same semantics as
+ * tdse_tdma per system. Replaces a loop of cy.tdma calls
+ * (ionization/cy.pyx:9-50), e.g. the decoupled lines of one block-diagonal
+ * TDMAOperator (ionization/mesh/mesh_operators.py:109-111).
+ */
+int tdse_tdma_batched(tdse_ctx *ctx, const double *sub, const double *diag, const double *sup, const double *d,
+                      double *x, const int64_t *offsets, const int64_t *lengths, int64_t n_systems);
+
+/*
+ * SphericalHarmonicMesh simulation, length gauge, SplitInteractionOperator
+ * (ionization/mesh/meshes.py:985-1136, mesh_operators.py:815-1080,
+ * evolution_methods.py:80-123). All arrays are in device position order:
+ * position p = i * chunks + k holds radial index j = k * rows + i,
+ * npad = chunks * rows.
+ */
+typedef struct {
+    int32_t r_points;      /* n */
+    int32_t l_bound;       /* L */
+    int32_t chunks;        /* P: threads per l-line, multiple of 64 */
+    int32_t rows;          /* m: rows per thread, even, 2 <= m <= 16 */
+    int32_t n_steps;       /* N = len(times) - 1 */
+    int32_t store_every;   /* data cadence (sims.py:84-95); -1 = first and last only */
+    int32_t n_basis;       /* test states for inner products */
+    int32_t lb;            /* rows l < lb are captured at data steps (>= 1 + max basis l) */
+    double delta_r;        /* inner-product weight (meshes.py:1013) */
+    double corr0;          /* tau0 (-2 P_r) dr (1 + dr) / 8 (mesh_operators.py:901-903) */
+} tdse_sh_desc;
+
+/*
+ * Add one SH simulation; it starts at step 0 on the next tdse_run. tables:
+ * 6 * npad doubles (T, U, A_{j-1}, A_j, tau0 (-q) r, mask), fs / ts: N
+ * doubles (field at t_{n+1} + dt/2 times tau_n / tau0, and tau_n / tau0),
+ * g0: 2 * L * npad doubles, basis: n_basis * npad doubles, basis_l: n_basis.
+ * Replaces MeshSpecification.to_sim() + the first call of
+ * SplitInteractionOperator.evolve (sims.py:46-118, evolution_methods.py:89-123).
+ */
+int tdse_sh_add(tdse_ctx *ctx, const tdse_sh_desc *desc, const double *tables, const double *fs, const double *ts,
+                const double *g0, const double *basis, const int32_t *basis_l, int32_t *slot);
+
+/*
+ * LineMesh simulation with the length-gauge Crank-Nicolson step
+ * (meshes.py:263-325, mesh_operators.py:304-327, evolution_methods.py:46-77).
+ * tables: 5 * npad doubles (H0 diagonal tau0-scaled, off-diagonal to j-1,
+ * off-diagonal to j+1, tau0 (-q) z, mask); wt: N complex (i tau_n / tau0);
+ * fs: N doubles (field at t_{n+1}); basis: n_basis * npad complex.
+ */
+typedef struct {
+    int32_t z_points;      /* n */
+    int32_t chunks;        /* P, multiple of 64, <= 1024 */
+    int32_t rows;          /* m, 2 <= m <= 16 */
+    int32_t n_steps;
+    int32_t store_every;
+    int32_t n_basis;
+    int32_t renormalize;   /* 1: renormalise after every step (imaginary-time relaxation) */
+    int32_t reserved;
+    double delta_z;
+} tdse_line_desc;
+
+int tdse_line_add(tdse_ctx *ctx, const tdse_line_desc *desc, const double *tables, const double *wt,
+                  const double *fs, const double *g0, const double *basis, int32_t *slot);
+
+/*
+ * Advance every active simulation by up to n_steps steps (MeshSimulation.run,
+ * sims.py:289-321). use_graph != 0 replays one captured hipGraph per step.
+ * Finished simulations stay resident until removed.
+ */
+int tdse_run(tdse_ctx *ctx, int32_t n_steps, int32_t use_graph);
+
+/* Advance only the LineMesh simulations, up to n_steps inside one launch. */
+int tdse_line_run(tdse_ctx *ctx, int32_t n_steps);
+
+/* kind: 0 = SH, 1 = Line. step: steps taken (n_steps + 1 once finished). */
+int tdse_sim_state(tdse_ctx *ctx, int32_t kind, int32_t slot, int32_t *step, int32_t *n_steps, int32_t *n_data);
+
+/*
+ * Copy a simulation's wavefunction (position order) and its stored data back
+ * to the host. Any pointer may be NULL to skip it. For SH: norms, lnorms
+ * (norm in l = L-1), ips (n_data * n_basis complex). For Line: lnorms is
+ * ignored. Replaces reading sim.mesh.g and the Norm / InnerProducts
+ * datastores (data.py:189-287).
+ */
+int tdse_download(tdse_ctx *ctx, int32_t kind, int32_t slot, double *g, double *norms, double *lnorms, double *ips);
+
+/* Overwrite a simulation's wavefunction (position order) and step counter (checkpoint resume). */
+int tdse_upload_state(tdse_ctx *ctx, int32_t kind, int32_t slot, const double *g, int32_t step);
+
+/* Release a simulation's slot and device memory (serving retirement). */
+int tdse_remove(tdse_ctx *ctx, int32_t kind, int32_t slot);
+
+/*
+ * Timing of the most recent tdse_run: total milliseconds between HIP events
+ * recorded on the context's stream around the whole run.
+ */
+int tdse_last_run_ms(tdse_ctx *ctx, double *ms);
+
+/*
+ * Per-kernel timing over the next tdse_run: when enabled, the run launches
+ * kernels directly (no graph) with HIP events around each L-phase and R-phase
+ * launch on the context's stream, and accumulates their durations.
+ */
+int tdse_profile(tdse_ctx *ctx, int32_t enable);
+int tdse_profile_read(tdse_ctx *ctx, double *lphase_ms, int64_t *lphase_launches, double *rphase_ms,
+                      int64_t *rphase_launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TDSE_HIP_H */

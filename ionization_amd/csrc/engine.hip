@@ -1,0 +1,800 @@
+// Native runtime of the mesh-TDSE library: context, device memory pool,
+// ragged work tables, hipGraph-captured steps and the C ABI of tdse_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/tdse_hip.h"
+#include "common.h"
+#include "line_kernels.h"
+#include "sh_kernels.h"
+#include "tdma_kernels.h"
+
+namespace {
+
+constexpr int kAbiVersion = 1;
+constexpr size_t kAlign = 256;
+constexpr int kMaxL = 4096;          // largest l_bound accepted (c_l table size)
+constexpr int kSlotCapacity = 8192;  // simulations of each kind per context
+
+thread_local std::string g_lib_error;
+
+// First-fit sub-allocator over one device buffer reserved at context creation.
+class Arena {
+   public:
+    char* base = nullptr;
+    size_t size = 0;
+    size_t used = 0;
+
+    bool init(size_t bytes) {
+        size = (bytes + kAlign - 1) / kAlign * kAlign;
+        if (hipMalloc(&base, size) != hipSuccess) {
+            base = nullptr;
+            return false;
+        }
+        free_[0] = size;
+        return true;
+    }
+    void release() {
+        if (base) (void)hipFree(base);
+        base = nullptr;
+    }
+    // Returns nullptr when no free block is large enough.
+    void* alloc(size_t bytes) {
+        bytes = std::max<size_t>(kAlign, (bytes + kAlign - 1) / kAlign * kAlign);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                const size_t off = it->first;
+                const size_t rest = it->second - bytes;
+                free_.erase(it);
+                if (rest) free_[off + bytes] = rest;
+                used_[off] = bytes;
+                used += bytes;
+                return base + off;
+            }
+        }
+        return nullptr;
+    }
+    void dealloc(void* p) {
+        if (!p) return;
+        const size_t off = static_cast<char*>(p) - base;
+        auto it = used_.find(off);
+        if (it == used_.end()) return;
+        size_t sz = it->second;
+        used -= sz;
+        used_.erase(it);
+        size_t start = off;
+        auto next = free_.lower_bound(off);
+        if (next != free_.end() && next->first == off + sz) {
+            sz += next->second;
+            free_.erase(next);
+        }
+        auto prev = free_.lower_bound(off);
+        if (prev != free_.begin()) {
+            --prev;
+            if (prev->first + prev->second == off) {
+                start = prev->first;
+                sz += prev->second;
+                free_.erase(prev);
+            }
+        }
+        free_[start] = sz;
+    }
+
+   private:
+    std::map<size_t, size_t> free_;
+    std::map<size_t, size_t> used_;
+};
+
+struct ShHost {
+    bool live = false;
+    ShSim dev{};
+    int n_data = 0;
+    std::vector<void*> blocks;
+};
+
+struct LineHost {
+    bool live = false;
+    LineSim dev{};
+    int n_data = 0;
+    std::vector<void*> blocks;
+};
+
+// Device table with host-side growth.
+template <class T>
+struct DevVec {
+    T* ptr = nullptr;
+    size_t cap = 0;
+    size_t count = 0;
+};
+
+struct Bucket {
+    int M = 0;
+    int P = 0;
+    DevVec<int2> lines;  // SH: (slot, l); Line: (slot, 0)
+};
+
+int n_data_points(int N, int every) {
+    if (every >= 1) return N / every + 1 + ((N % every) != 0 ? 1 : 0);
+    return 2;
+}
+
+}  // namespace
+
+struct tdse_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Arena arena;
+    std::string err;
+    std::vector<ShHost> sh;
+    std::vector<LineHost> line;
+    ShSim* d_sh = nullptr;
+    LineSim* d_line = nullptr;
+    double* d_cl = nullptr;
+    DevVec<int2> waves;
+    DevVec<int2> jobs;
+    std::vector<Bucket> sh_buckets;
+    std::vector<Bucket> line_buckets;
+    bool dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    double last_run_ms = 0.0;
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, int>> ev_pairs;  // (kind 0 = L-phase, 1 = R-phase, start event index)
+    double prof_ms[2] = {0.0, 0.0};
+    int64_t prof_n[2] = {0, 0};
+};
+
+namespace {
+
+int fail(tdse_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    else g_lib_error = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                          \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return fail((ctx), TDSE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+template <class T>
+int devvec_upload(tdse_ctx* ctx, DevVec<T>& v, const std::vector<T>& host) {
+    if (host.size() > v.cap) {
+        if (v.ptr) HIP_TRY(ctx, hipFree(v.ptr));
+        v.cap = std::max<size_t>(host.size(), 2 * v.cap);
+        HIP_TRY(ctx, hipMalloc(&v.ptr, v.cap * sizeof(T)));
+    }
+    v.count = host.size();
+    if (!host.empty())
+        HIP_TRY(ctx, hipMemcpyAsync(v.ptr, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    return TDSE_OK;
+}
+
+void drop_graph(tdse_ctx* ctx) {
+    if (ctx->exec) (void)hipGraphExecDestroy(ctx->exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->exec = nullptr;
+    ctx->graph = nullptr;
+}
+
+// Rebuild the ragged work tables from the live simulations.
+int rebuild_tables(tdse_ctx* ctx) {
+    std::vector<int2> waves, jobs;
+    std::map<std::pair<int, int>, std::vector<int2>> shb, lnb;
+    for (int s = 0; s < (int)ctx->sh.size(); ++s) {
+        const ShHost& h = ctx->sh[s];
+        if (!h.live) continue;
+        for (int p0 = 0; p0 < h.dev.npad; p0 += TDSE_WAVE) waves.push_back(make_int2(s, p0));
+        jobs.push_back(make_int2(s, -1));
+        for (int b = 0; b < h.dev.n_basis; ++b) jobs.push_back(make_int2(s, b));
+        const int M = h.dev.m <= 8 ? 8 : 16;
+        auto& v = shb[{M, h.dev.P}];
+        for (int l = 0; l < h.dev.L; ++l) v.push_back(make_int2(s, l));
+    }
+    for (int s = 0; s < (int)ctx->line.size(); ++s) {
+        const LineHost& h = ctx->line[s];
+        if (!h.live) continue;
+        const int M = h.dev.m <= 8 ? 8 : 16;
+        lnb[{M, h.dev.P}].push_back(make_int2(s, 0));
+    }
+    int rc;
+    if ((rc = devvec_upload(ctx, ctx->waves, waves))) return rc;
+    if ((rc = devvec_upload(ctx, ctx->jobs, jobs))) return rc;
+    auto fill = [&](std::vector<Bucket>& dst, std::map<std::pair<int, int>, std::vector<int2>>& src) -> int {
+        for (auto& b : dst)
+            if (b.lines.ptr) (void)hipFree(b.lines.ptr);
+        dst.clear();
+        for (auto& kv : src) {
+            Bucket b;
+            b.M = kv.first.first;
+            b.P = kv.first.second;
+            int r = devvec_upload(ctx, b.lines, kv.second);
+            if (r) return r;
+            dst.push_back(b);
+        }
+        return TDSE_OK;
+    };
+    if ((rc = fill(ctx->sh_buckets, shb))) return rc;
+    if ((rc = fill(ctx->line_buckets, lnb))) return rc;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    drop_graph(ctx);
+    ctx->dirty = false;
+    return TDSE_OK;
+}
+
+hipEvent_t next_event(tdse_ctx* ctx) {
+    if (ctx->ev_used == ctx->ev_pool.size()) {
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        ctx->ev_pool.push_back(e);
+    }
+    return ctx->ev_pool[ctx->ev_used++];
+}
+
+void prof_begin(tdse_ctx* ctx, int kind) {
+    if (!ctx->profiling) return;
+    ctx->ev_pairs.push_back({kind, (int)ctx->ev_used});
+    (void)hipEventRecord(next_event(ctx), ctx->stream);
+}
+void prof_end(tdse_ctx* ctx) {
+    if (!ctx->profiling) return;
+    (void)hipEventRecord(next_event(ctx), ctx->stream);
+}
+
+// Launch one full step of every simulation kind onto the context stream.
+int launch_step(tdse_ctx* ctx) {
+    const int nw = (int)ctx->waves.count;
+    if (nw > 0) {
+        prof_begin(ctx, 0);
+        hipLaunchKernelGGL(k_sh_lphase, dim3((nw * TDSE_WAVE + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh,
+                           ctx->waves.ptr, nw, ctx->d_cl);
+        prof_end(ctx);
+        const int nj = (int)ctx->jobs.count;
+        hipLaunchKernelGGL(k_sh_observe, dim3(nj), dim3(256), 0, ctx->stream, ctx->d_sh, ctx->jobs.ptr, nj);
+        for (auto& b : ctx->sh_buckets) {
+            const size_t lds = 3 * (size_t)b.P * sizeof(double2);
+            const int nl = (int)b.lines.count;
+            prof_begin(ctx, 1);
+            if (b.M == 8)
+                hipLaunchKernelGGL((k_sh_rphase<8, 512>), dim3(nl), dim3(b.P), lds, ctx->stream, ctx->d_sh,
+                                   b.lines.ptr, nl);
+            else
+                hipLaunchKernelGGL((k_sh_rphase<16, 256>), dim3(nl), dim3(b.P), lds, ctx->stream, ctx->d_sh,
+                                   b.lines.ptr, nl);
+            prof_end(ctx);
+        }
+        const int ns = (int)ctx->sh.size();
+        hipLaunchKernelGGL(k_sh_advance, dim3((ns + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh, ns);
+    }
+    for (auto& b : ctx->line_buckets) {
+        const size_t lds = 3 * (size_t)b.P * sizeof(double2) + 64 * sizeof(double);
+        const int nl = (int)b.lines.count;
+        if (b.M == 8)
+            hipLaunchKernelGGL((k_line_steps<8, 1024>), dim3(nl), dim3(b.P), lds, ctx->stream, ctx->d_line,
+                               b.lines.ptr, nl, 1);
+        else
+            hipLaunchKernelGGL((k_line_steps<16, 512>), dim3(nl), dim3(b.P), lds, ctx->stream, ctx->d_line,
+                               b.lines.ptr, nl, 1);
+    }
+    HIP_TRY(ctx, hipGetLastError());
+    return TDSE_OK;
+}
+
+int ensure_graph(tdse_ctx* ctx) {
+    if (ctx->exec) return TDSE_OK;
+    HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_step(ctx);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(ctx, TDSE_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    ctx->graph = g;
+    HIP_TRY(ctx, hipGraphInstantiate(&ctx->exec, ctx->graph, nullptr, nullptr, 0));
+    return TDSE_OK;
+}
+
+template <class T>
+int copy_to_device(tdse_ctx* ctx, std::vector<void*>& blocks, const T* host, size_t count, T** out) {
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    void* p = ctx->arena.alloc(bytes);
+    if (!p) return fail(ctx, TDSE_ERR_NOMEM, "device pool exhausted");
+    blocks.push_back(p);
+    if (host && count) HIP_TRY(ctx, hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    else HIP_TRY(ctx, hipMemsetAsync(p, 0, bytes, ctx->stream));
+    *out = static_cast<T*>(p);
+    return TDSE_OK;
+}
+
+void free_blocks(tdse_ctx* ctx, std::vector<void*>& blocks) {
+    for (void* p : blocks) ctx->arena.dealloc(p);
+    blocks.clear();
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+int tdse_abi_version(void) { return kAbiVersion; }
+
+const char* tdse_last_error(const tdse_ctx* ctx) { return ctx ? ctx->err.c_str() : g_lib_error.c_str(); }
+
+int tdse_ctx_create(int device, size_t pool_bytes, tdse_ctx** out) {
+    if (!out) return fail(nullptr, TDSE_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        return fail(nullptr, TDSE_ERR_HIP, "no HIP device available");
+    if (device < 0 || device >= count) return fail(nullptr, TDSE_ERR_ARG, "device index out of range");
+    if (hipSetDevice(device) != hipSuccess) return fail(nullptr, TDSE_ERR_HIP, "hipSetDevice failed");
+    tdse_ctx* ctx = new tdse_ctx();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return fail(nullptr, TDSE_ERR_HIP, "hipStreamCreate failed");
+    }
+    if (!ctx->arena.init(pool_bytes)) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return fail(nullptr, TDSE_ERR_NOMEM, "could not reserve the device memory pool");
+    }
+    std::vector<double> cl(kMaxL);
+    for (int l = 0; l < kMaxL; ++l) cl[l] = (l + 1.0) / std::sqrt((2.0 * l + 1.0) * (2.0 * l + 3.0));
+    bool ok = hipMalloc(&ctx->d_cl, kMaxL * sizeof(double)) == hipSuccess &&
+              hipMemcpy(ctx->d_cl, cl.data(), kMaxL * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMalloc(&ctx->d_sh, kSlotCapacity * sizeof(ShSim)) == hipSuccess &&
+              hipMalloc(&ctx->d_line, kSlotCapacity * sizeof(LineSim)) == hipSuccess &&
+              hipMemset(ctx->d_sh, 0, kSlotCapacity * sizeof(ShSim)) == hipSuccess &&
+              hipMemset(ctx->d_line, 0, kSlotCapacity * sizeof(LineSim)) == hipSuccess &&
+              hipEventCreate(&ctx->ev_start) == hipSuccess && hipEventCreate(&ctx->ev_stop) == hipSuccess;
+    if (!ok) {
+        tdse_ctx_destroy(ctx);
+        return fail(nullptr, TDSE_ERR_HIP, "context allocation failed");
+    }
+    *out = ctx;
+    return TDSE_OK;
+}
+
+int tdse_ctx_destroy(tdse_ctx* ctx) {
+    if (!ctx) return TDSE_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    drop_graph(ctx);
+    for (auto& b : ctx->sh_buckets)
+        if (b.lines.ptr) (void)hipFree(b.lines.ptr);
+    for (auto& b : ctx->line_buckets)
+        if (b.lines.ptr) (void)hipFree(b.lines.ptr);
+    if (ctx->waves.ptr) (void)hipFree(ctx->waves.ptr);
+    if (ctx->jobs.ptr) (void)hipFree(ctx->jobs.ptr);
+    if (ctx->d_cl) (void)hipFree(ctx->d_cl);
+    if (ctx->d_sh) (void)hipFree(ctx->d_sh);
+    if (ctx->d_line) (void)hipFree(ctx->d_line);
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+    if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    ctx->arena.release();
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return TDSE_OK;
+}
+
+int tdse_sync(tdse_ctx* ctx) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return TDSE_OK;
+}
+
+int tdse_pool_usage(tdse_ctx* ctx, size_t* used_bytes, size_t* total_bytes) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    if (used_bytes) *used_bytes = ctx->arena.used;
+    if (total_bytes) *total_bytes = ctx->arena.size;
+    return TDSE_OK;
+}
+
+int tdse_tdma_batched(tdse_ctx* ctx, const double* sub, const double* diag, const double* sup, const double* d,
+                      double* x, const int64_t* offsets, const int64_t* lengths, int64_t n_systems) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    if (!sub || !diag || !sup || !d || !x || !offsets || !lengths || n_systems < 0)
+        return fail(ctx, TDSE_ERR_ARG, "tdse_tdma_batched: NULL array or negative count");
+    if (n_systems == 0) return TDSE_OK;
+    int64_t total = 0;
+    for (int64_t i = 0; i < n_systems; ++i) {
+        if (lengths[i] < 1 || offsets[i] < 0) return fail(ctx, TDSE_ERR_ARG, "tdse_tdma_batched: bad system extent");
+        total = std::max(total, offsets[i] + lengths[i]);
+    }
+    std::vector<void*> blocks;
+    double2 *da, *db, *dc, *dd, *dx, *dcp;
+    int64_t *doff, *dlen;
+    int rc = 0;
+    const size_t n = (size_t)total;
+    if ((rc = copy_to_device(ctx, blocks, reinterpret_cast<const double2*>(sub), n, &da)) ||
+        (rc = copy_to_device(ctx, blocks, reinterpret_cast<const double2*>(diag), n, &db)) ||
+        (rc = copy_to_device(ctx, blocks, reinterpret_cast<const double2*>(sup), n, &dc)) ||
+        (rc = copy_to_device(ctx, blocks, reinterpret_cast<const double2*>(d), n, &dd)) ||
+        (rc = copy_to_device<double2>(ctx, blocks, nullptr, n, &dx)) ||
+        (rc = copy_to_device<double2>(ctx, blocks, nullptr, n, &dcp)) ||
+        (rc = copy_to_device(ctx, blocks, offsets, (size_t)n_systems, &doff)) ||
+        (rc = copy_to_device(ctx, blocks, lengths, (size_t)n_systems, &dlen))) {
+        (void)hipStreamSynchronize(ctx->stream);
+        free_blocks(ctx, blocks);
+        return rc;
+    }
+    const int threads = 64;
+    const int64_t grid = (n_systems + threads - 1) / threads;
+    hipLaunchKernelGGL(k_tdma_lines, dim3((unsigned)grid), dim3(threads), 0, ctx->stream, da, db, dc, dd, dx, dcp, doff,
+                       dlen, n_systems);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(x, dx, n * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_blocks(ctx, blocks);
+    if (e != hipSuccess) return fail(ctx, TDSE_ERR_HIP, std::string("tdse_tdma_batched: ") + hipGetErrorString(e));
+    return TDSE_OK;
+}
+
+int tdse_tdma(tdse_ctx* ctx, const double* sub, const double* diag, const double* sup, const double* d, double* x,
+              int64_t n) {
+    if (n < 1) return fail(ctx, TDSE_ERR_ARG, "tdse_tdma: n must be >= 1");
+    const int64_t off = 0;
+    return tdse_tdma_batched(ctx, sub, diag, sup, d, x, &off, &n, 1);
+}
+
+int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, const double* fs, const double* ts,
+                const double* g0, const double* basis, const int32_t* basis_l, int32_t* slot) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    if (!desc || !tables || !fs || !ts || !g0 || !slot) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: NULL argument");
+    const tdse_sh_desc& D = *desc;
+    const int P = D.chunks, m = D.rows;
+    if (D.r_points < 2 || D.l_bound < 1 || D.l_bound > kMaxL) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: bad mesh size");
+    if (P < TDSE_WAVE || P % TDSE_WAVE) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: chunks must be a positive multiple of 64");
+    if (m < 2 || m > 16 || (m & 1)) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: rows must be even and in [2, 16]");
+    if ((m <= 8 && P > 512) || (m > 8 && P > 256)) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: too many chunks for rows");
+    if ((int64_t)P * m < D.r_points) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: chunks * rows < r_points");
+    if (D.n_steps < 1) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: n_steps must be >= 1");
+    if (D.store_every == 0 || D.store_every < -1) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: store_every must be >= 1 or -1");
+    if (D.n_basis < 0 || (D.n_basis > 0 && (!basis || !basis_l))) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: basis missing");
+    if (D.lb < 1 || D.lb > D.l_bound) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: lb out of range");
+    for (int b = 0; b < D.n_basis; ++b)
+        if (basis_l[b] < 0 || basis_l[b] >= D.lb) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: basis l must be < lb");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int s = -1;
+    for (int i = 0; i < (int)ctx->sh.size(); ++i)
+        if (!ctx->sh[i].live) {
+            s = i;
+            break;
+        }
+    if (s < 0) {
+        if ((int)ctx->sh.size() >= kSlotCapacity) return fail(ctx, TDSE_ERR_NOMEM, "no free simulation slot");
+        ctx->sh.emplace_back();
+        s = (int)ctx->sh.size() - 1;
+    }
+    ShHost& H = ctx->sh[s];
+    H.blocks.clear();
+    ShSim& S = H.dev;
+    S = ShSim{};
+    S.n = D.r_points;
+    S.L = D.l_bound;
+    S.P = P;
+    S.m = m;
+    S.npad = P * m;
+    S.N = D.n_steps;
+    S.store_every = D.store_every;
+    S.LB = D.lb;
+    S.n_basis = D.n_basis;
+    S.nwaves = S.npad / TDSE_WAVE;
+    S.corr0 = D.corr0;
+    S.delta_r = D.delta_r;
+    S.step = 0;
+    S.active = 1;
+    H.n_data = n_data_points(S.N, S.store_every);
+    const size_t npad = S.npad;
+    double2 *g, *snap, *oip;
+    double *tab, *dfs, *dts, *part, *on, *oln, *bas = nullptr;
+    int* bl = nullptr;
+    int rc = 0;
+    if ((rc = copy_to_device(ctx, H.blocks, reinterpret_cast<const double2*>(g0), (size_t)S.L * npad, &g)) ||
+        (rc = copy_to_device(ctx, H.blocks, tables, SH_NTAB * npad, &tab)) ||
+        (rc = copy_to_device(ctx, H.blocks, fs, (size_t)S.N, &dfs)) ||
+        (rc = copy_to_device(ctx, H.blocks, ts, (size_t)S.N, &dts)) ||
+        (rc = copy_to_device<double2>(ctx, H.blocks, nullptr, (size_t)S.LB * npad, &snap)) ||
+        (rc = copy_to_device<double>(ctx, H.blocks, nullptr, 2 * (size_t)S.nwaves, &part)) ||
+        (rc = copy_to_device<double>(ctx, H.blocks, nullptr, (size_t)H.n_data, &on)) ||
+        (rc = copy_to_device<double>(ctx, H.blocks, nullptr, (size_t)H.n_data, &oln)) ||
+        (rc = copy_to_device<double2>(ctx, H.blocks, nullptr, (size_t)H.n_data * std::max(1, S.n_basis), &oip)) ||
+        (S.n_basis > 0 && ((rc = copy_to_device(ctx, H.blocks, basis, (size_t)S.n_basis * npad, &bas)) ||
+                           (rc = copy_to_device(ctx, H.blocks, basis_l, (size_t)S.n_basis, &bl))))) {
+        (void)hipStreamSynchronize(ctx->stream);
+        free_blocks(ctx, H.blocks);
+        return rc;
+    }
+    S.g = g;
+    S.tab = tab;
+    S.fs = dfs;
+    S.ts = dts;
+    S.snap = snap;
+    S.part = part;
+    S.out_norm = on;
+    S.out_lnorm = oln;
+    S.out_ip = oip;
+    S.basis = bas;
+    S.basis_l = bl;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->d_sh + s, &S, sizeof(ShSim), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // host buffers may be released after return
+    H.live = true;
+    ctx->dirty = true;
+    *slot = s;
+    return TDSE_OK;
+}
+
+int tdse_line_add(tdse_ctx* ctx, const tdse_line_desc* desc, const double* tables, const double* wt, const double* fs,
+                  const double* g0, const double* basis, int32_t* slot) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    if (!desc || !tables || !wt || !fs || !g0 || !slot) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: NULL argument");
+    const tdse_line_desc& D = *desc;
+    const int P = D.chunks, m = D.rows;
+    if (D.z_points < 2) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: z_points must be >= 2");
+    if (P < TDSE_WAVE || P % TDSE_WAVE) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: chunks must be a positive multiple of 64");
+    if (m < 2 || m > 16) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: rows must be in [2, 16]");
+    if ((m <= 8 && P > 1024) || (m > 8 && P > 512)) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: too many chunks for rows");
+    if ((int64_t)P * m < D.z_points) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: chunks * rows < z_points");
+    if (D.n_steps < 1) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: n_steps must be >= 1");
+    if (D.store_every == 0 || D.store_every < -1) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: store_every must be >= 1 or -1");
+    if (D.n_basis < 0 || (D.n_basis > 0 && !basis)) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: basis missing");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int s = -1;
+    for (int i = 0; i < (int)ctx->line.size(); ++i)
+        if (!ctx->line[i].live) {
+            s = i;
+            break;
+        }
+    if (s < 0) {
+        if ((int)ctx->line.size() >= kSlotCapacity) return fail(ctx, TDSE_ERR_NOMEM, "no free simulation slot");
+        ctx->line.emplace_back();
+        s = (int)ctx->line.size() - 1;
+    }
+    LineHost& H = ctx->line[s];
+    H.blocks.clear();
+    LineSim& S = H.dev;
+    S = LineSim{};
+    S.n = D.z_points;
+    S.P = P;
+    S.m = m;
+    S.npad = P * m;
+    S.N = D.n_steps;
+    S.store_every = D.store_every;
+    S.n_basis = D.n_basis;
+    S.renormalize = D.renormalize;
+    S.delta_z = D.delta_z;
+    S.step = 0;
+    S.active = 1;
+    H.n_data = n_data_points(S.N, S.store_every);
+    const size_t npad = S.npad;
+    double2 *g, *dwt, *oip, *bas = nullptr;
+    double *tab, *dfs, *on;
+    int rc = 0;
+    if ((rc = copy_to_device(ctx, H.blocks, reinterpret_cast<const double2*>(g0), npad, &g)) ||
+        (rc = copy_to_device(ctx, H.blocks, tables, LN_NTAB * npad, &tab)) ||
+        (rc = copy_to_device(ctx, H.blocks, reinterpret_cast<const double2*>(wt), (size_t)S.N, &dwt)) ||
+        (rc = copy_to_device(ctx, H.blocks, fs, (size_t)S.N, &dfs)) ||
+        (rc = copy_to_device<double>(ctx, H.blocks, nullptr, (size_t)H.n_data, &on)) ||
+        (rc = copy_to_device<double2>(ctx, H.blocks, nullptr, (size_t)H.n_data * std::max(1, S.n_basis), &oip)) ||
+        (S.n_basis > 0 &&
+         (rc = copy_to_device(ctx, H.blocks, reinterpret_cast<const double2*>(basis), (size_t)S.n_basis * npad, &bas)))) {
+        (void)hipStreamSynchronize(ctx->stream);
+        free_blocks(ctx, H.blocks);
+        return rc;
+    }
+    S.g = g;
+    S.tab = tab;
+    S.wt = dwt;
+    S.fs = dfs;
+    S.out_norm = on;
+    S.out_ip = oip;
+    S.basis = bas;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->d_line + s, &S, sizeof(LineSim), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // host buffers may be released after return
+    H.live = true;
+    ctx->dirty = true;
+    *slot = s;
+    return TDSE_OK;
+}
+
+int tdse_run(tdse_ctx* ctx, int32_t n_steps, int32_t use_graph) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    if (n_steps < 0) return fail(ctx, TDSE_ERR_ARG, "tdse_run: n_steps must be >= 0");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int rc;
+    if (ctx->dirty && (rc = rebuild_tables(ctx))) return rc;
+    const bool graph = use_graph && !ctx->profiling;
+    if (graph && (rc = ensure_graph(ctx))) return rc;
+    ctx->ev_used = 0;
+    ctx->ev_pairs.clear();
+    HIP_TRY(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
+    for (int i = 0; i < n_steps; ++i) {
+        if (graph) HIP_TRY(ctx, hipGraphLaunch(ctx->exec, ctx->stream));
+        else if ((rc = launch_step(ctx))) return rc;
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev_stop));
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    ctx->last_run_ms = ms;
+    if (ctx->profiling) {
+        for (auto& pr : ctx->ev_pairs) {
+            float t = 0.f;
+            HIP_TRY(ctx, hipEventElapsedTime(&t, ctx->ev_pool[pr.second], ctx->ev_pool[pr.second + 1]));
+            ctx->prof_ms[pr.first] += t;
+            ctx->prof_n[pr.first] += 1;
+        }
+    }
+    return TDSE_OK;
+}
+
+int tdse_line_run(tdse_ctx* ctx, int32_t n_steps) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    if (n_steps < 0) return fail(ctx, TDSE_ERR_ARG, "tdse_line_run: n_steps must be >= 0");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int rc;
+    if (ctx->dirty && (rc = rebuild_tables(ctx))) return rc;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
+    for (auto& b : ctx->line_buckets) {
+        const size_t lds = 3 * (size_t)b.P * sizeof(double2) + 64 * sizeof(double);
+        const int nl = (int)b.lines.count;
+        if (b.M == 8)
+            hipLaunchKernelGGL((k_line_steps<8, 1024>), dim3(nl), dim3(b.P), lds, ctx->stream, ctx->d_line,
+                               b.lines.ptr, nl, n_steps);
+        else
+            hipLaunchKernelGGL((k_line_steps<16, 512>), dim3(nl), dim3(b.P), lds, ctx->stream, ctx->d_line,
+                               b.lines.ptr, nl, n_steps);
+    }
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev_stop));
+    float ms = 0.f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    ctx->last_run_ms = ms;
+    return TDSE_OK;
+}
+
+int tdse_sim_state(tdse_ctx* ctx, int32_t kind, int32_t slot, int32_t* step, int32_t* n_steps, int32_t* n_data) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    int st = 0;
+    if (kind == 0) {
+        if (slot < 0 || slot >= (int)ctx->sh.size() || !ctx->sh[slot].live) return fail(ctx, TDSE_ERR_ARG, "bad SH slot");
+        HIP_TRY(ctx, hipMemcpy(&st, reinterpret_cast<char*>(ctx->d_sh + slot) + offsetof(ShSim, step), sizeof(int),
+                               hipMemcpyDeviceToHost));
+        if (n_steps) *n_steps = ctx->sh[slot].dev.N;
+        if (n_data) *n_data = ctx->sh[slot].n_data;
+    } else if (kind == 1) {
+        if (slot < 0 || slot >= (int)ctx->line.size() || !ctx->line[slot].live)
+            return fail(ctx, TDSE_ERR_ARG, "bad Line slot");
+        HIP_TRY(ctx, hipMemcpy(&st, reinterpret_cast<char*>(ctx->d_line + slot) + offsetof(LineSim, step), sizeof(int),
+                               hipMemcpyDeviceToHost));
+        if (n_steps) *n_steps = ctx->line[slot].dev.N;
+        if (n_data) *n_data = ctx->line[slot].n_data;
+    } else {
+        return fail(ctx, TDSE_ERR_ARG, "unknown simulation kind");
+    }
+    if (step) *step = st;
+    return TDSE_OK;
+}
+
+int tdse_download(tdse_ctx* ctx, int32_t kind, int32_t slot, double* g, double* norms, double* lnorms, double* ips) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (kind == 0) {
+        if (slot < 0 || slot >= (int)ctx->sh.size() || !ctx->sh[slot].live) return fail(ctx, TDSE_ERR_ARG, "bad SH slot");
+        const ShHost& H = ctx->sh[slot];
+        const ShSim& S = H.dev;
+        if (g) HIP_TRY(ctx, hipMemcpy(g, S.g, (size_t)S.L * S.npad * sizeof(double2), hipMemcpyDeviceToHost));
+        if (norms) HIP_TRY(ctx, hipMemcpy(norms, S.out_norm, H.n_data * sizeof(double), hipMemcpyDeviceToHost));
+        if (lnorms) HIP_TRY(ctx, hipMemcpy(lnorms, S.out_lnorm, H.n_data * sizeof(double), hipMemcpyDeviceToHost));
+        if (ips && S.n_basis > 0)
+            HIP_TRY(ctx, hipMemcpy(ips, S.out_ip, (size_t)H.n_data * S.n_basis * sizeof(double2), hipMemcpyDeviceToHost));
+    } else if (kind == 1) {
+        if (slot < 0 || slot >= (int)ctx->line.size() || !ctx->line[slot].live)
+            return fail(ctx, TDSE_ERR_ARG, "bad Line slot");
+        const LineHost& H = ctx->line[slot];
+        const LineSim& S = H.dev;
+        if (g) HIP_TRY(ctx, hipMemcpy(g, S.g, (size_t)S.npad * sizeof(double2), hipMemcpyDeviceToHost));
+        if (norms) HIP_TRY(ctx, hipMemcpy(norms, S.out_norm, H.n_data * sizeof(double), hipMemcpyDeviceToHost));
+        if (ips && S.n_basis > 0)
+            HIP_TRY(ctx, hipMemcpy(ips, S.out_ip, (size_t)H.n_data * S.n_basis * sizeof(double2), hipMemcpyDeviceToHost));
+    } else {
+        return fail(ctx, TDSE_ERR_ARG, "unknown simulation kind");
+    }
+    return TDSE_OK;
+}
+
+int tdse_upload_state(tdse_ctx* ctx, int32_t kind, int32_t slot, const double* g, int32_t step) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    if (!g) return fail(ctx, TDSE_ERR_ARG, "g is NULL");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (kind == 0) {
+        if (slot < 0 || slot >= (int)ctx->sh.size() || !ctx->sh[slot].live) return fail(ctx, TDSE_ERR_ARG, "bad SH slot");
+        const ShSim& S = ctx->sh[slot].dev;
+        if (step < 0 || step > S.N + 1) return fail(ctx, TDSE_ERR_ARG, "step out of range");
+        HIP_TRY(ctx, hipMemcpy(S.g, g, (size_t)S.L * S.npad * sizeof(double2), hipMemcpyHostToDevice));
+        HIP_TRY(ctx, hipMemcpy(reinterpret_cast<char*>(ctx->d_sh + slot) + offsetof(ShSim, step), &step, sizeof(int),
+                               hipMemcpyHostToDevice));
+    } else if (kind == 1) {
+        if (slot < 0 || slot >= (int)ctx->line.size() || !ctx->line[slot].live)
+            return fail(ctx, TDSE_ERR_ARG, "bad Line slot");
+        const LineSim& S = ctx->line[slot].dev;
+        if (step < 0 || step > S.N + 1) return fail(ctx, TDSE_ERR_ARG, "step out of range");
+        HIP_TRY(ctx, hipMemcpy(S.g, g, (size_t)S.npad * sizeof(double2), hipMemcpyHostToDevice));
+        HIP_TRY(ctx, hipMemcpy(reinterpret_cast<char*>(ctx->d_line + slot) + offsetof(LineSim, step), &step, sizeof(int),
+                               hipMemcpyHostToDevice));
+    } else {
+        return fail(ctx, TDSE_ERR_ARG, "unknown simulation kind");
+    }
+    return TDSE_OK;
+}
+
+int tdse_remove(tdse_ctx* ctx, int32_t kind, int32_t slot) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    const int zero = 0;
+    if (kind == 0) {
+        if (slot < 0 || slot >= (int)ctx->sh.size() || !ctx->sh[slot].live) return fail(ctx, TDSE_ERR_ARG, "bad SH slot");
+        HIP_TRY(ctx, hipMemcpy(reinterpret_cast<char*>(ctx->d_sh + slot) + offsetof(ShSim, active), &zero, sizeof(int),
+                               hipMemcpyHostToDevice));
+        free_blocks(ctx, ctx->sh[slot].blocks);
+        ctx->sh[slot].live = false;
+    } else if (kind == 1) {
+        if (slot < 0 || slot >= (int)ctx->line.size() || !ctx->line[slot].live)
+            return fail(ctx, TDSE_ERR_ARG, "bad Line slot");
+        HIP_TRY(ctx, hipMemcpy(reinterpret_cast<char*>(ctx->d_line + slot) + offsetof(LineSim, active), &zero,
+                               sizeof(int), hipMemcpyHostToDevice));
+        free_blocks(ctx, ctx->line[slot].blocks);
+        ctx->line[slot].live = false;
+    } else {
+        return fail(ctx, TDSE_ERR_ARG, "unknown simulation kind");
+    }
+    ctx->dirty = true;
+    return TDSE_OK;
+}
+
+int tdse_last_run_ms(tdse_ctx* ctx, double* ms) {
+    if (!ctx || !ms) return fail(ctx, TDSE_ERR_ARG, "NULL argument");
+    *ms = ctx->last_run_ms;
+    return TDSE_OK;
+}
+
+int tdse_profile(tdse_ctx* ctx, int32_t enable) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    ctx->profiling = enable != 0;
+    ctx->prof_ms[0] = ctx->prof_ms[1] = 0.0;
+    ctx->prof_n[0] = ctx->prof_n[1] = 0;
+    return TDSE_OK;
+}
+
+int tdse_profile_read(tdse_ctx* ctx, double* lphase_ms, int64_t* lphase_launches, double* rphase_ms,
+                      int64_t* rphase_launches) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    if (lphase_ms) *lphase_ms = ctx->prof_ms[0];
+    if (lphase_launches) *lphase_launches = ctx->prof_n[0];
+    if (rphase_ms) *rphase_ms = ctx->prof_ms[1];
+    if (rphase_launches) *rphase_launches = ctx->prof_n[1];
+    return TDSE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,148 @@
+// LineMesh Crank-Nicolson on gfx950.
+//
+// One reference step (AlternatingDirectionImplicit with the single total
+// Hamiltonian of a LineMesh, evolution_methods.py:46-77 and
+// mesh_operators.py:271-327; then the mask, meshes.py:251-257):
+//     g <- M . (1 + i tau H)^-1 (1 - i tau H) g,   H = K + V(z) - z q E(t_{n+1})
+// with K the 3-point kinetic stencil (diagonal -2 P_z, off-diagonal P_z),
+// tau = (t_{n+1} - t_n) / 2 hbar (complex tau for imaginary-time relaxation)
+// and M the mask evaluated at the signed z (meshes.py:318-320).
+//
+// One workgroup owns one simulation's line for the whole launch: every thread
+// holds its chunk of m rows in registers and the kernel advances up to
+// n_steps steps before writing g back, so a whole run can be one launch.
+// Layout: position p = i P + k holds z-index j = k m + i (as for SH lines).
+#pragma once
+
+#include "common.h"
+#include "line_solver.h"
+
+struct LineSim {
+    double2* g;            // [npad]
+    const double* tab;     // [LINE_NTAB][npad]: H0 diagonal, off-diag to j-1, off-diag to j+1, z coupling, mask
+    const double2* wt;     // [N] w_n = i tau_n / tau0 (complex for imaginary time)
+    const double* fs;      // [N] field E(t_{n+1})
+    double* out_norm;      // [n_data]
+    double2* out_ip;       // [n_data][n_basis]
+    const double2* basis;  // [n_basis][npad] test states' g, position order
+    double delta_z;
+    int n, P, m, npad, N, store_every, n_basis, renormalize;
+    int step;
+    int active;
+};
+
+enum { LN_H = 0, LN_AP = 1, LN_AN = 2, LN_ZQ = 3, LN_MASK = 4, LN_NTAB = 5 };
+
+__device__ __forceinline__ bool line_is_data_step(int n, int N, int every) {
+    return n == 0 || n == N || (every >= 1 && (n % every) == 0);
+}
+__device__ __forceinline__ int line_data_slot(int n, int N, int every) {
+    if (every >= 1) return (n == N && (N % every) != 0) ? N / every + 1 : n / every;
+    return n == 0 ? 0 : 1;
+}
+
+// Block-wide sum in a fixed order (deterministic for a given block size).
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    const int t = threadIdx.x;
+    v = wave_sum(v);
+    __syncthreads();
+    if ((t % TDSE_WAVE) == 0) red[t / TDSE_WAVE] = v;
+    __syncthreads();
+    double tot = 0.0;
+    const int nw = (blockDim.x + TDSE_WAVE - 1) / TDSE_WAVE;
+    for (int w = 0; w < nw; ++w) tot += red[w];
+    return tot;
+}
+
+template <int M>
+__device__ __forceinline__ void line_observe(const LineSim& S, const double2 (&x)[M], int m, int k, int P,
+                                             int slot, double* red) {
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+        if (i < m) acc += cabs2(x[i]);
+    const double nrm = block_sum(acc, red);
+    if (threadIdx.x == 0) S.out_norm[slot] = nrm * S.delta_z;
+    for (int b = 0; b < S.n_basis; ++b) {
+        const double2* phi = S.basis + (size_t)b * S.npad + k;
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+            if (i < m) {
+                const double2 f = phi[i * P];
+                // conj(f) * x
+                re += f.x * x[i].x + f.y * x[i].y;
+                im += f.x * x[i].y - f.y * x[i].x;
+            }
+        const double sre = block_sum(re, red);
+        const double sim = block_sum(im, red);
+        if (threadIdx.x == 0) S.out_ip[(size_t)slot * S.n_basis + b] = cmk(sre * S.delta_z, sim * S.delta_z);
+    }
+}
+
+// blockDim.x == P of every simulation launched together; one block per simulation.
+template <int M, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_line_steps(LineSim* __restrict__ sims, const int2* __restrict__ ids,
+                                                     int n_ids, int n_steps) {
+    extern __shared__ double2 lds[];  // 3 * P double2, then reduction scratch
+    if ((int)blockIdx.x >= n_ids) return;
+    LineSim& S = sims[ids[blockIdx.x].x];
+    if (!S.active) return;
+    int n = S.step;
+    const int N = S.N;
+    if (n > N) return;
+    const int P = S.P, m = S.m, npad = S.npad;
+    const int k = threadIdx.x;
+    double* red = reinterpret_cast<double*>(lds + 3 * P);
+    double2* __restrict__ g = S.g + k;
+    const double* __restrict__ H = S.tab + LN_H * npad + k;
+    const double* __restrict__ AP = S.tab + LN_AP * npad + k;
+    const double* __restrict__ AN = S.tab + LN_AN * npad + k;
+    const double* __restrict__ ZQ = S.tab + LN_ZQ * npad + k;
+    const double* __restrict__ MK = S.tab + LN_MASK * npad + k;
+
+    double2 x[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+        if (i < m) x[i] = g[(size_t)i * P];
+
+    if (n == 0 && line_is_data_step(0, N, S.store_every)) line_observe<M>(S, x, m, k, P, 0, red);
+
+    for (int it = 0; it < n_steps && n < N; ++it, ++n) {
+        const double2 w = S.wt[n];
+        const double f = S.fs[n];
+        double2 xlast = x[0];
+#pragma unroll
+        for (int i = 1; i < M; ++i)
+            if (i < m) xlast = x[i];
+        double2 xl, xr;
+        chunk_neighbours<M>(x, xlast, k, P, lds, xl, xr, true);
+        auto coef = [&](int i, double& hm, double& h, double& hp) {
+            h = fma(f, ZQ[i * P], H[i * P]);
+            hm = AP[i * P];
+            hp = AN[i * P];
+        };
+        cn_partition<M, false>(x, m, k, P, xl, xr, w, coef, lds, true);
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+            if (i < m) x[i] = cscale(MK[i * P], x[i]);
+        if (S.renormalize) {
+            // imaginary-time relaxation: restore unit norm after every step
+            double acc = 0.0;
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+                if (i < m) acc += cabs2(x[i]);
+            const double inv = 1.0 / sqrt(block_sum(acc, red) * S.delta_z);
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+                if (i < m) x[i] = cscale(inv, x[i]);
+        }
+        if (line_is_data_step(n + 1, N, S.store_every))
+            line_observe<M>(S, x, m, k, P, line_data_slot(n + 1, N, S.store_every), red);
+    }
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+        if (i < m) g[(size_t)i * P] = x[i];
+    __syncthreads();
+    if (k == 0) S.step = (n == N) ? N + 1 : n;
+}

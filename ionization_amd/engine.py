@@ -1,0 +1,452 @@
+"""
+Device engine: compiles simulations into the device's table format and
+drives them through libtdse_hip.so.
+
+A ``DeviceContext`` owns one HIP stream and one device memory pool. Any
+number of simulations of different kinds and sizes live in it at once; one
+``run`` call advances all of them by a number of steps (one captured
+hipGraph replay per step). This is the batched path; the "plain" path is the
+same context holding a single simulation.
+
+Device layout of every line (an l-line of a SphericalHarmonicMesh, or a
+LineMesh): position p = i * P + k holds radial / z index j = k * m + i, with
+P = 64 w chunk-threads of m rows and npad = P * m >= n. The helpers
+``to_positions`` / ``from_positions`` convert between that layout and the
+reference's natural order.
+"""
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Callable, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from . import units as u
+from .exceptions import DeviceError
+
+KIND_SH = 0
+KIND_LINE = 1
+
+SH_NTAB = 6
+LINE_NTAB = 5
+
+
+# ----------------------------------------------------------------- layout
+
+
+def choose_layout(n: int, kind: str = "sh", rows: Optional[int] = None) -> Tuple[int, int]:
+    """
+    Pick (P, m) for a line of n points: P a multiple of 64, m even, P m >= n,
+    within the kernel limits (SH: m <= 8 -> P <= 512, m <= 16 -> P <= 256;
+    Line: m <= 8 -> P <= 1024, m <= 16 -> P <= 512). Prefers m = 8, the
+    variant with the lighter register footprint, and the least padding.
+    """
+    limits = {"sh": {8: 512, 16: 256}, "line": {8: 1024, 16: 512}}[kind]
+    cands = [rows] if rows is not None else [8, 6, 4, 16, 14, 12, 10, 2]
+    best = None
+    for m in cands:
+        if m < 2 or m % 2:
+            continue
+        cap = limits[8] if m <= 8 else limits[16]
+        P = 64 * -(-n // (64 * m))
+        if P > cap:
+            continue
+        key = (P * m - n, 0 if m == 8 else 1, -m)
+        if best is None or key < best[0]:
+            best = (key, P, m)
+    if best is None:
+        raise DeviceError(f"no device layout for a line of {n} points")
+    return best[1], best[2]
+
+
+def position_index(n: int, P: int, m: int) -> np.ndarray:
+    """j for every position p (j >= n marks padding)."""
+    p = np.arange(P * m)
+    return (p % P) * m + p // P
+
+
+def to_positions(a: np.ndarray, P: int, m: int) -> np.ndarray:
+    """Natural order (last axis, length n) -> position order (length P m, zero padded)."""
+    n = a.shape[-1]
+    j = position_index(n, P, m)
+    out = np.zeros(a.shape[:-1] + (P * m,), dtype=a.dtype)
+    valid = j < n
+    out[..., valid] = a[..., j[valid]]
+    return out
+
+
+def from_positions(a: np.ndarray, n: int, P: int, m: int) -> np.ndarray:
+    """Position order -> natural order of length n."""
+    j = position_index(n, P, m)
+    out = np.zeros(a.shape[:-1] + (n,), dtype=a.dtype)
+    valid = j < n
+    out[..., j[valid]] = a[..., valid]
+    return out
+
+
+def data_count(n_steps: int, store_every: int) -> int:
+    if store_every >= 1:
+        return n_steps // store_every + 1 + (1 if n_steps % store_every else 0)
+    return 2
+
+
+def data_indices(n_steps: int, store_every: int) -> np.ndarray:
+    """Time indices at which data is stored (sims.py:84-95)."""
+    idx = np.arange(n_steps + 1)
+    mask = (idx == 0) | (idx == n_steps)
+    if store_every >= 1:
+        mask |= idx % store_every == 0
+    return idx[mask]
+
+
+# ----------------------------------------------------------------- compiled simulations
+
+
+@dataclass
+class CompiledSH:
+    """One SphericalHarmonicMesh simulation in device table form."""
+
+    r_points: int
+    l_bound: int
+    P: int
+    m: int
+    n_steps: int
+    store_every: int
+    lb: int
+    delta_r: float
+    corr0: float
+    tables: np.ndarray  # (6, npad)
+    fs: np.ndarray  # (N,)
+    ts: np.ndarray  # (N,)
+    g0: np.ndarray  # (L, npad) complex128, position order
+    basis: np.ndarray  # (nb, npad) float64, position order
+    basis_l: np.ndarray  # (nb,) int32
+
+    @property
+    def npad(self):
+        return self.P * self.m
+
+    @property
+    def mesh_points(self):
+        return self.r_points * self.l_bound
+
+    @property
+    def footprint_bytes(self):
+        """Device bytes this simulation holds (g, tables, snapshots, basis, data)."""
+        nd = data_count(self.n_steps, self.store_every)
+        return int(
+            16 * self.l_bound * self.npad
+            + 8 * SH_NTAB * self.npad
+            + 16 * self.n_steps
+            + 16 * self.lb * self.npad
+            + 8 * len(self.basis_l) * self.npad
+            + nd * (16 + 16 * max(1, len(self.basis_l)))
+        )
+
+
+def sh_radial_tables(*, r_bound, r_points, potential, mask, time_step, test_charge=u.electron_charge,
+                     hydrogen_correction=True, P, m):
+    """
+    Radial tables of SphericalHarmonicLengthGaugeOperators in tau0 units,
+    tau0 = time_step / (2 hbar), position order:
+      T_j = tau0 (beta_j (-2 P_r) + V(r_j))               (mesh_operators.py:898-928, 244-269)
+      U_j = tau0 hbar^2 / (2 mu r_j^2)                     (centrifugal factor per l(l+1))
+      A_j = tau0 P_r alpha_j (j < n-1), stored as A_{j-1} and A_j
+      RQ_j = tau0 (-q) r_j                                 (mesh_operators.py:988-1006)
+      MASK_j = mask(r_j)                                   (masks.py:76-89)
+    with P_r = -hbar^2 / (2 mu dr^2), the reduced mass mu regardless of test_mass
+    (mesh_operators.py:892-894). Returns (tables, corr0, delta_r, r).
+    """
+    r = np.linspace(0, r_bound, r_points)
+    delta_r = r[1] - r[0]
+    r = r + delta_r / 2
+    mu = u.electron_mass_reduced
+    tau0 = time_step / (2 * u.hbar)
+    pr = -(u.hbar ** 2) / (2 * mu * (delta_r ** 2))
+    j = np.arange(r_points, dtype=np.float64)
+    xb = 2 * (j ** 2) + (2 * j)
+    beta = (xb + 1) / (xb + 0.5)
+    xa = (j ** 2) + (2 * j)
+    alpha = (xa + 1) / (xa + 0.75)
+    v = np.asarray(potential(r), dtype=np.float64) * np.ones_like(r)
+    T = tau0 * (beta * (-2 * pr) + v)
+    U = tau0 * ((u.hbar ** 2) / (2 * mu)) / (r ** 2)
+    A = tau0 * pr * alpha
+    A[-1] = 0.0
+    AP = np.concatenate([[0.0], A[:-1]])
+    RQ = tau0 * (-test_charge) * r
+    MK = np.asarray(mask(r), dtype=np.float64) * np.ones_like(r) if mask is not None else np.ones_like(r)
+    corr0 = 0.0
+    if hydrogen_correction:
+        dr = delta_r / u.bohr_radius
+        corr0 = tau0 * (-2 * pr) * (dr * (1 + dr) / 8)
+    tables = to_positions(np.stack([T, U, AP, A, RQ, MK]), P, m)
+    return tables, corr0, delta_r, r
+
+
+def compile_sh(*, r_bound, r_points, l_bound, potential, field: Callable, mask, times, time_step,
+               g0: np.ndarray, basis: Sequence[Tuple[np.ndarray, int]] = (), store_every=1,
+               test_charge=u.electron_charge, hydrogen_correction=True, rows=None) -> CompiledSH:
+    """
+    Compile a length-gauge SphericalHarmonicMesh + SplitInteractionOperator
+    simulation. ``times`` is the reference time grid (sims.py:198-205);
+    step n uses tau_n = (t_{n+1} - t_n) / 2 hbar and the field at
+    t_{n+1} + time_step / 2 (mesh_operators.py:1011-1013).
+    """
+    P, m = choose_layout(r_points, "sh", rows)
+    tables, corr0, delta_r, _ = sh_radial_tables(
+        r_bound=r_bound, r_points=r_points, potential=potential, mask=mask, time_step=time_step,
+        test_charge=test_charge, hydrogen_correction=hydrogen_correction, P=P, m=m,
+    )
+    times = np.asarray(times, dtype=np.float64)
+    steps = np.diff(times)
+    ts = steps / time_step
+    fs = np.asarray(field(times[1:] + time_step / 2), dtype=np.float64) * np.ones_like(ts) * ts
+    g0 = np.asarray(g0, dtype=np.complex128).reshape(l_bound, r_points)
+    lb = 1 + max([l for _, l in basis], default=0)
+    bmat = np.zeros((len(basis), r_points), dtype=np.float64)
+    bl = np.zeros(len(basis), dtype=np.int32)
+    for i, (vec, l) in enumerate(basis):
+        if np.iscomplexobj(vec) and np.any(np.imag(vec) != 0):
+            raise DeviceError("SH basis states must be real radial functions")
+        bmat[i] = np.real(vec)
+        bl[i] = l
+    return CompiledSH(
+        r_points=r_points, l_bound=l_bound, P=P, m=m, n_steps=len(steps), store_every=int(store_every),
+        lb=min(lb, l_bound), delta_r=delta_r, corr0=corr0, tables=np.ascontiguousarray(tables),
+        fs=np.ascontiguousarray(fs), ts=np.ascontiguousarray(ts),
+        g0=np.ascontiguousarray(to_positions(g0, P, m)),
+        basis=np.ascontiguousarray(to_positions(bmat, P, m)), basis_l=bl,
+    )
+
+
+@dataclass
+class CompiledLine:
+    """One LineMesh simulation in device table form."""
+
+    z_points: int
+    P: int
+    m: int
+    n_steps: int
+    store_every: int
+    delta_z: float
+    renormalize: bool
+    tables: np.ndarray  # (5, npad)
+    wt: np.ndarray  # (N,) complex: i tau_n / tau0
+    fs: np.ndarray  # (N,)
+    g0: np.ndarray  # (npad,) complex
+    basis: np.ndarray  # (nb, npad) complex
+
+    @property
+    def npad(self):
+        return self.P * self.m
+
+    @property
+    def mesh_points(self):
+        return self.z_points
+
+    @property
+    def footprint_bytes(self):
+        nd = data_count(self.n_steps, self.store_every)
+        return int(16 * self.npad * (1 + len(self.basis)) + 8 * LINE_NTAB * self.npad + 24 * self.n_steps
+                   + nd * 16 * (1 + max(1, len(self.basis))))
+
+
+def compile_line(*, z_bound, z_points, potential, field: Callable, mask, times, g0,
+                 basis: Sequence[np.ndarray] = (), store_every=1, test_mass=u.electron_mass_reduced,
+                 test_charge=u.electron_charge, time_steps=None, renormalize=False, rows=None) -> CompiledLine:
+    """
+    Compile a LineMesh + LineLengthGaugeOperators + ADI simulation
+    (meshes.py:263-325, mesh_operators.py:304-327). H = K + V(z) - z q E(t_{n+1});
+    tau0 = (t_1 - t_0) / 2 hbar. ``time_steps`` (complex) overrides the real
+    steps of ``times`` (imaginary-time relaxation); the mask is evaluated at
+    the signed z (meshes.py:318-320).
+    """
+    P, m = choose_layout(z_points, "line", rows)
+    z = np.linspace(-z_bound, z_bound, z_points)
+    delta_z = np.abs(z[1] - z[0])
+    times = np.asarray(times, dtype=np.float64)
+    steps = np.diff(times).astype(np.complex128) if time_steps is None else np.asarray(time_steps, np.complex128)
+    tau0 = abs(steps[0]) / (2 * u.hbar)
+    pz = -(u.hbar ** 2) / (2 * test_mass * (delta_z ** 2))
+    v = np.asarray(potential(z), dtype=np.float64) * np.ones_like(z)
+    H = tau0 * ((-2 * pz) + v)
+    AN = np.full(z_points, tau0 * pz)
+    AN[-1] = 0.0
+    AP = np.full(z_points, tau0 * pz)
+    AP[0] = 0.0
+    ZQ = tau0 * (-z * test_charge)
+    MK = np.asarray(mask(z), dtype=np.float64) * np.ones_like(z) if mask is not None else np.ones_like(z)
+    tables = to_positions(np.stack([H, AP, AN, ZQ, MK]), P, m)
+    wt = 1j * (steps / (2 * u.hbar)) / tau0
+    fs = np.asarray(field(times[1:]), dtype=np.float64) * np.ones(len(steps))
+    bmat = np.array([np.asarray(b, dtype=np.complex128) for b in basis]).reshape(len(basis), z_points)
+    return CompiledLine(
+        z_points=z_points, P=P, m=m, n_steps=len(steps), store_every=int(store_every), delta_z=delta_z,
+        renormalize=bool(renormalize), tables=np.ascontiguousarray(tables), wt=np.ascontiguousarray(wt),
+        fs=np.ascontiguousarray(fs), g0=np.ascontiguousarray(to_positions(np.asarray(g0, np.complex128), P, m)),
+        basis=np.ascontiguousarray(to_positions(bmat, P, m)),
+    )
+
+
+# ----------------------------------------------------------------- context
+
+
+@dataclass
+class ShResult:
+    g: np.ndarray  # (L, n) natural order
+    norms: np.ndarray
+    lnorms: np.ndarray
+    ips: np.ndarray  # (n_data, nb)
+
+
+class DeviceContext:
+    """One tdse_ctx: a HIP stream and a device memory pool holding many simulations."""
+
+    def __init__(self, device: int = 0, pool_bytes: int = 4 << 30):
+        self._lib = _lib.load()
+        handle = ctypes.c_void_p()
+        _lib.check(self._lib.tdse_ctx_create(int(device), ctypes.c_size_t(int(pool_bytes)), ctypes.byref(handle)))
+        self._ctx = handle
+        self.pool_bytes = int(pool_bytes)
+        self._sh = {}
+        self._line = {}
+
+    # -- lifetime
+    def close(self):
+        if self._ctx:
+            self._lib.tdse_ctx_destroy(self._ctx)
+            self._ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        _lib.check(rc, self._ctx)
+
+    def sync(self):
+        self._check(self._lib.tdse_sync(self._ctx))
+
+    def pool_usage(self):
+        used, total = ctypes.c_size_t(), ctypes.c_size_t()
+        self._check(self._lib.tdse_pool_usage(self._ctx, ctypes.byref(used), ctypes.byref(total)))
+        return used.value, total.value
+
+    # -- op level
+    def tdma(self, sub, diag, sup, d):
+        """x = T^-1 d (cy.tdma semantics); sub[i] couples x[i-1], sup[i] couples x[i+1]."""
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128) for a in (sub, diag, sup, d)]
+        n = len(arrs[3])
+        x = np.empty(n, dtype=np.complex128)
+        self._check(self._lib.tdse_tdma(self._ctx, *[_lib.dptr(a) for a in arrs], _lib.dptr(x), n))
+        return x
+
+    def tdma_batched(self, sub, diag, sup, d, offsets, lengths):
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128) for a in (sub, diag, sup, d)]
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        ln = np.ascontiguousarray(lengths, dtype=np.int64)
+        x = np.zeros(len(arrs[3]), dtype=np.complex128)
+        self._check(self._lib.tdse_tdma_batched(
+            self._ctx, *[_lib.dptr(a) for a in arrs], _lib.dptr(x),
+            off.ctypes.data_as(_lib.c_int64_p), ln.ctypes.data_as(_lib.c_int64_p), len(off)))
+        return x
+
+    # -- simulations
+    def add_sh(self, c: CompiledSH) -> int:
+        desc = _lib.ShDesc(c.r_points, c.l_bound, c.P, c.m, c.n_steps, c.store_every, len(c.basis_l), c.lb,
+                           c.delta_r, c.corr0)
+        slot = ctypes.c_int32()
+        basis = np.ascontiguousarray(c.basis, dtype=np.float64) if len(c.basis_l) else None
+        bl = np.ascontiguousarray(c.basis_l, dtype=np.int32)
+        self._check(self._lib.tdse_sh_add(
+            self._ctx, ctypes.byref(desc), _lib.dptr(c.tables), _lib.dptr(c.fs), _lib.dptr(c.ts), _lib.dptr(c.g0),
+            _lib.dptr(basis), bl.ctypes.data_as(_lib.c_int32_p) if len(bl) else None, ctypes.byref(slot)))
+        self._sh[slot.value] = c
+        return slot.value
+
+    def add_line(self, c: CompiledLine) -> int:
+        desc = _lib.LineDesc(c.z_points, c.P, c.m, c.n_steps, c.store_every, len(c.basis), int(c.renormalize), 0,
+                             c.delta_z)
+        slot = ctypes.c_int32()
+        basis = np.ascontiguousarray(c.basis, dtype=np.complex128) if len(c.basis) else None
+        self._check(self._lib.tdse_line_add(
+            self._ctx, ctypes.byref(desc), _lib.dptr(c.tables), _lib.dptr(c.wt), _lib.dptr(c.fs), _lib.dptr(c.g0),
+            _lib.dptr(basis), ctypes.byref(slot)))
+        self._line[slot.value] = c
+        return slot.value
+
+    def run(self, n_steps: int, use_graph: bool = True) -> float:
+        """Advance every simulation by n_steps steps; returns the device time in ms."""
+        self._check(self._lib.tdse_run(self._ctx, int(n_steps), 1 if use_graph else 0))
+        return self.last_run_ms()
+
+    def line_run(self, n_steps: int) -> float:
+        self._check(self._lib.tdse_line_run(self._ctx, int(n_steps)))
+        return self.last_run_ms()
+
+    def last_run_ms(self) -> float:
+        ms = ctypes.c_double()
+        self._check(self._lib.tdse_last_run_ms(self._ctx, ctypes.byref(ms)))
+        return ms.value
+
+    def state(self, kind: int, slot: int):
+        st, nn, nd = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        self._check(self._lib.tdse_sim_state(self._ctx, kind, slot, ctypes.byref(st), ctypes.byref(nn),
+                                             ctypes.byref(nd)))
+        return st.value, nn.value, nd.value
+
+    def finished(self, kind: int, slot: int) -> bool:
+        st, nn, _ = self.state(kind, slot)
+        return st > nn
+
+    def download_sh(self, slot: int, with_g: bool = True) -> ShResult:
+        c = self._sh[slot]
+        _, _, nd = self.state(KIND_SH, slot)
+        g = np.empty((c.l_bound, c.npad), dtype=np.complex128) if with_g else None
+        norms = np.empty(nd)
+        lnorms = np.empty(nd)
+        ips = np.empty((nd, max(1, len(c.basis_l))), dtype=np.complex128)
+        self._check(self._lib.tdse_download(self._ctx, KIND_SH, slot, _lib.dptr(g), _lib.dptr(norms),
+                                            _lib.dptr(lnorms), _lib.dptr(ips)))
+        gn = from_positions(g, c.r_points, c.P, c.m) if with_g else None
+        return ShResult(gn, norms, lnorms, ips[:, : len(c.basis_l)])
+
+    def download_line(self, slot: int):
+        c = self._line[slot]
+        _, _, nd = self.state(KIND_LINE, slot)
+        g = np.empty(c.npad, dtype=np.complex128)
+        norms = np.empty(nd)
+        ips = np.empty((nd, max(1, len(c.basis))), dtype=np.complex128)
+        self._check(self._lib.tdse_download(self._ctx, KIND_LINE, slot, _lib.dptr(g), _lib.dptr(norms), None,
+                                            _lib.dptr(ips)))
+        return from_positions(g, c.z_points, c.P, c.m), norms, ips[:, : len(c.basis)]
+
+    def upload_state(self, kind: int, slot: int, g_natural: np.ndarray, step: int):
+        c = self._sh[slot] if kind == KIND_SH else self._line[slot]
+        gp = np.ascontiguousarray(to_positions(np.asarray(g_natural, np.complex128), c.P, c.m))
+        self._check(self._lib.tdse_upload_state(self._ctx, kind, slot, _lib.dptr(gp), int(step)))
+
+    def remove(self, kind: int, slot: int):
+        self._check(self._lib.tdse_remove(self._ctx, kind, slot))
+        (self._sh if kind == KIND_SH else self._line).pop(slot, None)
+
+    def profile(self, enable: bool):
+        self._check(self._lib.tdse_profile(self._ctx, 1 if enable else 0))
+
+    def profile_read(self):
+        lm, rm = ctypes.c_double(), ctypes.c_double()
+        ln, rn = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self._lib.tdse_profile_read(self._ctx, ctypes.byref(lm), ctypes.byref(ln), ctypes.byref(rm),
+                                                ctypes.byref(rn)))
+        return {"lphase_ms": lm.value, "lphase_launches": ln.value, "rphase_ms": rm.value,
+                "rphase_launches": rn.value}

@@ -1,0 +1,403 @@
+"""
+Host-side physics inputs of the evolve step: static potentials V(r), the
+time-dependent field amplitude E(t), time windows and absorbing masks.
+
+These run on the host only. The device path consumes them as per-simulation
+tables (E sampled at every step, V and the mask sampled on the radial grid),
+built once when a simulation is compiled for the device.
+"""
+
+import numpy as np
+
+from .. import units as u
+from .. import exceptions
+from ..summables import Summand, Sum
+
+
+# ---------------------------------------------------------------- potentials
+
+
+class PotentialEnergy(Summand):
+    """A potential energy term; ``+`` builds a PotentialEnergySum (``potentials/potential.py:14-20``)."""
+
+    @property
+    def summation_class(self):
+        return PotentialEnergySum
+
+
+class PotentialEnergySum(Sum, PotentialEnergy):
+    container_name = "potentials"
+
+    def get_electric_field_amplitude(self, t):
+        return sum(x.get_electric_field_amplitude(t) for x in self._container)
+
+
+class NoPotentialEnergy(PotentialEnergy):
+    def __call__(self, *args, **kwargs):
+        return 0
+
+
+class CoulombPotential(PotentialEnergy):
+    """k q Q / r (``potentials/static.py:12-38``)."""
+
+    def __init__(self, charge: float = 1 * u.proton_charge):
+        self.charge = charge
+
+    def __call__(self, *, r, test_charge, **kwargs):
+        return u.coulomb_constant * self.charge * test_charge / r
+
+    def __repr__(self):
+        return f"CoulombPotential(charge={self.charge / u.proton_charge} e)"
+
+
+class SoftCoulombPotential(PotentialEnergy):
+    """k q Q / sqrt(r^2 + s^2) (``potentials/static.py:54-91``)."""
+
+    def __init__(
+        self,
+        charge: float = 1 * u.proton_charge,
+        softening_distance: float = 0.05 * u.bohr_radius,
+    ):
+        self.charge = charge
+        self.softening_distance = softening_distance
+
+    def __call__(self, *, r, test_charge, **kwargs):
+        return (
+            u.coulomb_constant
+            * self.charge
+            * test_charge
+            / np.sqrt((r ** 2) + (self.softening_distance ** 2))
+        )
+
+
+# ---------------------------------------------------------------- windows
+
+
+class TimeWindow(Summand):
+    @property
+    def summation_class(self):
+        return TimeWindowSum
+
+
+class TimeWindowSum(Sum, TimeWindow):
+    container_name = "windows"
+    combine = staticmethod(lambda values: np.prod(np.broadcast_arrays(*values), axis=0))
+
+
+class NoTimeWindow(TimeWindow):
+    def __call__(self, t):
+        return 1
+
+
+class RectangularWindow(TimeWindow):
+    """1 on [start, end], else 0 (``potentials/windows.py:36-70``)."""
+
+    def __init__(self, start_time: float = 0 * u.asec, end_time: float = 50 * u.asec):
+        self.start_time = start_time
+        self.end_time = end_time
+
+    def __call__(self, t):
+        cond = np.greater_equal(t, self.start_time) * np.less_equal(t, self.end_time)
+        return np.where(cond, 1, 0)
+
+
+class LogisticWindow(TimeWindow):
+    """Difference of two logistic steps (``potentials/windows.py:129-168``)."""
+
+    def __init__(self, *, window_time: float, window_width: float, window_center: float = 0.0):
+        self.window_time = window_time
+        self.window_width = window_width
+        self.window_center = window_center
+
+    def __call__(self, t):
+        tau = np.array(t) - self.window_center
+        return (1 / (1 + np.exp(-(tau + self.window_time) / self.window_width))) - (
+            1 / (1 + np.exp(-(tau - self.window_time) / self.window_width))
+        )
+
+
+class SmoothedTrapezoidalWindow(TimeWindow):
+    """sin^2 ramp up over ``time_front``, plateau, cos^2 ramp down (``potentials/windows.py:186-225``)."""
+
+    def __init__(self, *, time_front: float, time_plateau: float):
+        self.time_front = time_front
+        self.time_plateau = time_plateau
+
+    def __call__(self, t):
+        cond_before = np.less(t, self.time_front)
+        cond_middle = np.less_equal(self.time_front, t) * np.less_equal(
+            t, self.time_front + self.time_plateau
+        )
+        cond_after = np.less(self.time_front + self.time_plateau, t) * np.less_equal(
+            t, (2 * self.time_front) + self.time_plateau
+        )
+        out = np.where(cond_before, np.sin(u.pi * t / (2 * self.time_front)) ** 2, 0)
+        out += np.where(cond_middle, 1, 0)
+        out += np.where(
+            cond_after,
+            np.cos(u.pi * (t - (self.time_front + self.time_plateau)) / (2 * self.time_front)) ** 2,
+            0,
+        )
+        return out
+
+
+# ---------------------------------------------------------------- fields
+
+
+class ElectricPotential(PotentialEnergy):
+    pass
+
+
+class UniformLinearlyPolarizedElectricPotential(ElectricPotential):
+    """
+    A spatially uniform field along z. Its potential energy is -z q E(t)
+    (``potentials/pulses.py:23-56``); subclasses give E(t) times the window.
+    """
+
+    def __init__(self, window: TimeWindow = None):
+        self.window = NoTimeWindow() if window is None else window
+
+    def get_electric_field_amplitude(self, t):
+        return self.window(t)
+
+    def __call__(self, *, t, z, test_charge, **kwargs):
+        return -z * test_charge * self.get_electric_field_amplitude(t)
+
+
+class NoElectricPotential(UniformLinearlyPolarizedElectricPotential):
+    def get_electric_field_amplitude(self, t):
+        return np.zeros(np.shape(t)) * super().get_electric_field_amplitude(t)
+
+
+class SineWave(UniformLinearlyPolarizedElectricPotential):
+    """E(t) = A sin(w t + phase) * window (``potentials/pulses.py:215-246, 419-425``)."""
+
+    def __init__(self, omega: float, amplitude: float = 1 * u.atomic_electric_field, phase=0, **kwargs):
+        if omega <= 0:
+            raise exceptions.InvalidPotentialParameter("omega must be positive")
+        super().__init__(**kwargs)
+        self.omega = omega
+        self.phase = phase % u.twopi
+        self.amplitude = amplitude
+
+    @property
+    def period(self):
+        return u.twopi / self.omega
+
+    def get_electric_field_amplitude(self, t):
+        return np.sin((self.omega * t) + self.phase) * self.amplitude * super().get_electric_field_amplitude(t)
+
+
+DEFAULT_PULSE_WIDTH = 200 * u.asec
+DEFAULT_FLUENCE = 1 * u.Jcm2
+DEFAULT_PHASE = 0
+DEFAULT_OMEGA_MIN = u.twopi * 30 * u.THz
+DEFAULT_OMEGA_CARRIER = u.twopi * 2530 * u.THz
+DEFAULT_PULSE_CENTER = 0 * u.asec
+
+
+def sinc(x):
+    """sin(x)/x (``potentials/pulses.py:594-596``)."""
+    return np.sinc(x / u.pi)
+
+
+class SincPulse(UniformLinearlyPolarizedElectricPotential):
+    """
+    E(t) = A sinc(dw tau / 2) cos(w_c tau + phase) * window, with dw = 2 pi / pulse_width,
+    w_c = w_min + dw / 2 and A = sqrt(F dw / (pi eps0 c))
+    (``potentials/pulses.py:646-695, 929-940``).
+    """
+
+    def __init__(
+        self,
+        pulse_width: float = DEFAULT_PULSE_WIDTH,
+        fluence: float = DEFAULT_FLUENCE,
+        phase: float = DEFAULT_PHASE,
+        pulse_center: float = DEFAULT_PULSE_CENTER,
+        omega_min: float = DEFAULT_OMEGA_MIN,
+        **kwargs,
+    ):
+        if pulse_width <= 0:
+            raise exceptions.InvalidPotentialParameter("pulse width must be positive")
+        if fluence < 0:
+            raise exceptions.InvalidPotentialParameter("fluence must be non-negative")
+        if omega_min <= 0:
+            raise exceptions.InvalidPotentialParameter("omega_min must be positive")
+        super().__init__(**kwargs)
+        self.omega_min = omega_min
+        self.pulse_width = pulse_width
+        self.phase = phase % u.twopi
+        self.fluence = fluence
+        self.pulse_center = pulse_center
+        self.delta_omega = u.twopi / self.pulse_width
+        self.omega_max = self.omega_min + self.delta_omega
+        self.omega_carrier = (self.omega_min + self.omega_max) / 2
+        self.amplitude = np.sqrt(self.fluence * self.delta_omega / (u.pi * u.epsilon_0 * u.c))
+
+    def get_electric_field_envelope(self, t):
+        tau = np.array(t) - self.pulse_center
+        return sinc(self.delta_omega * tau / 2)
+
+    def get_electric_field_amplitude(self, t):
+        tau = np.array(t) - self.pulse_center
+        amp = self.get_electric_field_envelope(t) * np.cos((self.omega_carrier * tau) + self.phase)
+        return amp * self.amplitude * super().get_electric_field_amplitude(t)
+
+    def __repr__(self):
+        return (
+            f"SincPulse(pulse_width={self.pulse_width / u.asec:.6g} as, "
+            f"fluence={self.fluence / u.Jcm2:.6g} J/cm^2, phase={self.phase:.6g})"
+        )
+
+
+class GaussianPulse(UniformLinearlyPolarizedElectricPotential):
+    """
+    E(t) = A exp(-tau^2 / (2 pw^2)) cos(w_c tau + phase) * window, with
+    A = sqrt(2 F / (sqrt(pi) eps0 c pw)) (``potentials/pulses.py:1016-1060, 1361-1372``).
+    """
+
+    def __init__(
+        self,
+        pulse_width=DEFAULT_PULSE_WIDTH,
+        omega_carrier=DEFAULT_OMEGA_CARRIER,
+        fluence=DEFAULT_FLUENCE,
+        phase=DEFAULT_PHASE,
+        pulse_center=DEFAULT_PULSE_CENTER,
+        **kwargs,
+    ):
+        if pulse_width <= 0:
+            raise exceptions.InvalidPotentialParameter("pulse width must be positive")
+        if fluence < 0:
+            raise exceptions.InvalidPotentialParameter("fluence must be non-negative")
+        if omega_carrier < 0:
+            raise exceptions.InvalidPotentialParameter("omega_carrier must be non-negative")
+        super().__init__(**kwargs)
+        self.omega_carrier = omega_carrier
+        self.pulse_width = pulse_width
+        self.phase = phase % u.twopi
+        self.fluence = fluence
+        self.pulse_center = pulse_center
+        self.delta_omega = 1 / pulse_width
+        self.amplitude = np.sqrt(2 * self.fluence / (np.sqrt(u.pi) * u.epsilon_0 * u.c * self.pulse_width))
+
+    @classmethod
+    def fluence_from_pulse_width_and_amplitude(cls, pulse_width, amplitude):
+        return np.sqrt(u.pi) * pulse_width * u.epsilon_0 * u.c * (amplitude ** 2) / 2
+
+    @classmethod
+    def from_omega_min(
+        cls,
+        pulse_width=DEFAULT_PULSE_WIDTH,
+        omega_min=DEFAULT_OMEGA_MIN,
+        fluence=DEFAULT_FLUENCE,
+        phase=DEFAULT_PHASE,
+        pulse_center=DEFAULT_PULSE_CENTER,
+        **kwargs,
+    ):
+        """Carrier taken from a SincPulse of the same width and omega_min (``pulses.py:1064-1100``)."""
+        dummy = SincPulse(pulse_width=pulse_width, omega_min=omega_min)
+        pulse = cls(
+            pulse_width=pulse_width,
+            omega_carrier=dummy.omega_carrier,
+            fluence=fluence,
+            phase=phase,
+            pulse_center=pulse_center,
+            **kwargs,
+        )
+        pulse.omega_min = omega_min
+        return pulse
+
+    @classmethod
+    def from_amplitude(
+        cls,
+        pulse_width=DEFAULT_PULSE_WIDTH,
+        omega_min=DEFAULT_OMEGA_MIN,
+        amplitude=1 * u.atomic_electric_field,
+        phase=DEFAULT_PHASE,
+        pulse_center=DEFAULT_PULSE_CENTER,
+        **kwargs,
+    ):
+        """Fluence derived from the amplitude, then the amplitude pinned (``pulses.py:1151-1175``)."""
+        fluence = cls.fluence_from_pulse_width_and_amplitude(pulse_width, amplitude)
+        pot = cls.from_omega_min(
+            pulse_width=pulse_width,
+            omega_min=omega_min,
+            fluence=fluence,
+            phase=phase,
+            pulse_center=pulse_center,
+            **kwargs,
+        )
+        if np.isclose(amplitude / pot.amplitude, 1):
+            pot.amplitude = amplitude
+        else:
+            raise ValueError("Given amplitude not close enough to calculated amplitude")
+        return pot
+
+    def get_electric_field_envelope(self, t):
+        tau = np.array(t) - self.pulse_center
+        return np.exp(-0.5 * ((tau / self.pulse_width) ** 2))
+
+    def get_electric_field_amplitude(self, t):
+        tau = t - self.pulse_center
+        amp = self.get_electric_field_envelope(t) * np.cos((self.omega_carrier * tau) + self.phase)
+        return amp * self.amplitude * super().get_electric_field_amplitude(t)
+
+
+# ---------------------------------------------------------------- masks
+
+
+class Mask(Summand):
+    @property
+    def summation_class(self):
+        return MaskSum
+
+
+class MaskSum(Sum, Mask):
+    """Masks multiply (``potentials/masks.py:19-28``)."""
+
+    container_name = "masks"
+    combine = staticmethod(lambda values: np.prod(np.broadcast_arrays(*values), axis=0))
+
+
+class NoMask(Mask):
+    def __call__(self, *args, **kwargs):
+        return 1
+
+
+class RadialCosineMask(Mask):
+    """
+    1 for r < r_in; |cos(pi/2 (r - r_in)/(r_out - r_in))|^(1/s) on [r_in, r_out); 0 beyond
+    (``potentials/masks.py:38-89``).
+    """
+
+    def __init__(
+        self,
+        inner_radius: float = 50 * u.bohr_radius,
+        outer_radius: float = 100 * u.bohr_radius,
+        smoothness: float = 8,
+    ):
+        if inner_radius < 0 or outer_radius < 0:
+            raise exceptions.InvalidMaskParameter("inner and outer radius must be non-negative")
+        if inner_radius >= outer_radius:
+            raise exceptions.InvalidMaskParameter("outer radius must be larger than inner radius")
+        if smoothness < 1:
+            raise exceptions.InvalidMaskParameter("smoothness must be greater than 1")
+        self.inner_radius = inner_radius
+        self.outer_radius = outer_radius
+        self.smoothness = smoothness
+
+    def __call__(self, *, r, **kwargs):
+        return np.where(
+            np.greater_equal(r, self.inner_radius) * np.less(r, self.outer_radius),
+            np.abs(
+                np.cos(0.5 * u.pi * (r - self.inner_radius) / np.abs(self.outer_radius - self.inner_radius))
+            )
+            ** (1 / self.smoothness),
+            np.where(np.greater_equal(r, self.outer_radius), 0, 1),
+        )
+
+    def __repr__(self):
+        return (
+            f"RadialCosineMask(inner_radius={self.inner_radius / u.bohr_radius:.6g} a0, "
+            f"outer_radius={self.outer_radius / u.bohr_radius:.6g} a0, smoothness={self.smoothness})"
+        )

@@ -1,0 +1,151 @@
+"""
+Parity of the HIP path against the CPU oracle on the same inputs.
+
+Tolerances (from the task's parity bar): final wavefunction relative L2
+<= 1e-10; norm, bound-state overlap and ionization probability <= 1e-9
+absolute. The TDMA drop-in is checked like tests/test_tdma.py:12-26 of the
+reference (dense inverse, and the result solves the system).
+"""
+
+import numpy as np
+import pytest
+import scipy.sparse as sparse
+
+from oracle import mesh_cpu
+from tests.cases import line_case, sh_case
+from tests.conftest import complex_random_sample, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+WF_RTOL = 1e-10
+OBS_ATOL = 1e-9
+
+
+@pytest.mark.parametrize("n", [2, 3, 17, 64, 1000])
+def test_tdma_agrees_with_solution_via_inverse(device_ctx, n):
+    rng = np.random.default_rng(n)
+    a = complex_random_sample(n - 1, rng)
+    b = complex_random_sample(n, rng)
+    c = complex_random_sample(n - 1, rng)
+    d = complex_random_sample(n, rng)
+    dia = sparse.diags([a, b, c], offsets=[-1, 0, 1])
+    inv_x = np.linalg.inv(dia.toarray()).dot(d)
+    sub = np.concatenate([[0], a])
+    sup = np.concatenate([c, [0]])
+    x = device_ctx.tdma(sub, b, sup, d)
+    assert np.allclose(dia.dot(inv_x), d)
+    assert np.allclose(x, inv_x)
+    assert np.allclose(dia.dot(x), d)
+    # and against the oracle's restatement of cy.tdma
+    assert rel_l2(x, mesh_cpu.tdma(dia.todia(), d)) < 1e-10
+
+
+def test_tdma_batched_ragged_matches_single(device_ctx):
+    rng = np.random.default_rng(7)
+    lengths = np.array([2, 5, 1, 300, 64, 33])
+    offsets = np.concatenate([[0], np.cumsum(lengths)[:-1]])
+    tot = int(lengths.sum())
+    sub, diag, sup, d = (complex_random_sample(tot, rng) for _ in range(4))
+    diag += 4.0  # diagonally dominant
+    x = device_ctx.tdma_batched(sub, diag, sup, d, offsets, lengths)
+    for o, n in zip(offsets, lengths):
+        s = slice(o, o + n)
+        T = np.diag(diag[s]) + np.diag(sub[s][1:], -1) + np.diag(sup[s][:-1], 1)
+        assert rel_l2(x[s], np.linalg.solve(T, d[s])) < 1e-12
+
+
+def _run_sh_device(ctx, comp):
+    slot = ctx.add_sh(comp)
+    ctx.run(comp.n_steps + 1)
+    assert ctx.finished(0, slot)
+    res = ctx.download_sh(slot)
+    ctx.remove(0, slot)
+    return res
+
+
+@pytest.mark.parametrize(
+    "r_points, l_bound",
+    [(200, 20), (201, 21), (150, 7), (257, 12)],
+)
+def test_sh_split_operator_matches_oracle(device_ctx, r_points, l_bound):
+    case = sh_case(r_points=r_points, l_bound=l_bound, t_half_as=150.0)
+    print(f"SH case r_points={r_points} l_bound={l_bound} steps={case['compiled'].n_steps}", flush=True)
+    res = _run_sh_device(device_ctx, case["compiled"])
+    g, norms, ips, lnorm = case["oracle"].run(case["g0"], case["times"], case["dt"], case["store_every"],
+                                             basis=case["basis"])
+    assert rel_l2(res.g, g) <= WF_RTOL
+    assert np.max(np.abs(res.norms - norms)) <= OBS_ATOL
+    assert np.max(np.abs(res.lnorms - lnorm)) <= OBS_ATOL
+    assert np.max(np.abs(res.ips - ips)) <= OBS_ATOL
+    bound_dev = np.sum(np.abs(res.ips[-1]) ** 2)
+    bound_orc = np.sum(np.abs(ips[-1]) ** 2)
+    assert abs(bound_dev - bound_orc) <= OBS_ATOL
+    assert abs((1 - bound_dev) - (1 - bound_orc)) <= OBS_ATOL
+
+
+def test_sh_store_every_and_sparse_data(device_ctx):
+    case = sh_case(r_points=120, l_bound=10, t_half_as=60.0, store_every=7)
+    res = _run_sh_device(device_ctx, case["compiled"])
+    g, norms, ips, lnorm = case["oracle"].run(case["g0"], case["times"], case["dt"], 7, basis=case["basis"])
+    assert len(res.norms) == len(norms)
+    assert rel_l2(res.g, g) <= WF_RTOL
+    assert np.max(np.abs(res.norms - norms)) <= OBS_ATOL
+    assert np.max(np.abs(res.ips - ips)) <= OBS_ATOL
+
+
+def test_sh_alone_equals_in_batch(device_ctx):
+    """A simulation gives bit-identical results alone and in a batch of uneven neighbours."""
+    a = sh_case(r_points=200, l_bound=20, t_half_as=80.0)["compiled"]
+    alone = _run_sh_device(device_ctx, a)
+    others = [sh_case(r_points=rp, l_bound=lb, t_half_as=th, fluence_jcm2=fl)["compiled"]
+              for rp, lb, th, fl in [(333, 15, 40.0, 3.0), (90, 9, 120.0, 0.1), (501, 31, 20.0, 1.0)]]
+    slots = [device_ctx.add_sh(o) for o in others[:2]]
+    sa = device_ctx.add_sh(a)
+    slots.append(device_ctx.add_sh(others[2]))
+    device_ctx.run(max(c.n_steps for c in [a] + others) + 1)
+    mixed = device_ctx.download_sh(sa)
+    for s in slots + [sa]:
+        device_ctx.remove(0, s)
+    assert np.array_equal(mixed.g, alone.g)
+    assert np.array_equal(mixed.norms, alone.norms)
+    assert np.array_equal(mixed.ips, alone.ips)
+
+
+def test_sh_graph_and_direct_launch_agree(device_ctx):
+    comp = sh_case(r_points=130, l_bound=11, t_half_as=30.0)["compiled"]
+    s = device_ctx.add_sh(comp)
+    device_ctx.run(comp.n_steps + 1, use_graph=False)
+    direct = device_ctx.download_sh(s)
+    device_ctx.remove(0, s)
+    graph = _run_sh_device(device_ctx, comp)
+    assert np.array_equal(direct.g, graph.g)
+
+
+@pytest.mark.parametrize("z_points", [500, 257])
+def test_line_cn_matches_oracle(device_ctx, z_points):
+    case = line_case(z_points=z_points, n_steps=200)
+    comp = case["compiled"]
+    slot = device_ctx.add_line(comp)
+    device_ctx.line_run(comp.n_steps)
+    assert device_ctx.finished(1, slot)
+    g_dev, norms_dev, ips_dev = device_ctx.download_line(slot)
+    device_ctx.remove(1, slot)
+    g, norms, ips = case["oracle"].run(case["g0"], case["times"], case["store_every"], basis=case["basis"])
+    assert rel_l2(g_dev, g) <= WF_RTOL
+    assert np.max(np.abs(norms_dev - norms)) <= OBS_ATOL
+    assert np.max(np.abs(ips_dev - ips)) <= OBS_ATOL
+
+
+def test_line_stepwise_equals_single_launch(device_ctx):
+    case = line_case(z_points=300, n_steps=50)
+    comp = case["compiled"]
+    s1 = device_ctx.add_line(comp)
+    device_ctx.line_run(comp.n_steps)
+    g1, n1, _ = device_ctx.download_line(s1)
+    device_ctx.remove(1, s1)
+    s2 = device_ctx.add_line(comp)
+    device_ctx.run(comp.n_steps)  # one step per graph replay
+    g2, n2, _ = device_ctx.download_line(s2)
+    device_ctx.remove(1, s2)
+    assert np.array_equal(g1, g2)
+    assert np.array_equal(n1, n2)
