@@ -144,7 +144,20 @@ int tdse_sim_state(tdse_ctx *ctx, int32_t kind, int32_t slot, int32_t *step, int
  */
 int tdse_download(tdse_ctx *ctx, int32_t kind, int32_t slot, double *g, double *norms, double *lnorms, double *ips);
 
-/* Overwrite a simulation's wavefunction (position order) and step counter (checkpoint resume). */
+/*
+ * Make the device copy of an SH simulation's wavefunction equal to the
+ * reference g after its current step count (the step pipeline otherwise keeps
+ * the last half of step n-1 fused with the first half of step n). Needed
+ * before tdse_download when a caller inspects g mid-run: the per-step
+ * QuantumMesh.evolve / callback / checkpoint points of MeshSimulation.run
+ * (ionization/mesh/meshes.py:251-257, sims.py:296-330). No-op for LineMesh.
+ */
+int tdse_materialize(tdse_ctx *ctx, int32_t kind, int32_t slot);
+
+/*
+ * Overwrite a simulation's wavefunction (position order; the reference g after
+ * `step` steps) and its step counter: checkpoint resume (sims.py:277-279, 348-356).
+ */
 int tdse_upload_state(tdse_ctx *ctx, int32_t kind, int32_t slot, const double *g, int32_t step);
 
 /* Release a simulation's slot and device memory (serving retirement). */

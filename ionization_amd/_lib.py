@@ -84,6 +84,7 @@ SIGNATURES = {
         [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, c_double_p, ctypes.c_int32],
     ),
     "tdse_remove": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "tdse_materialize": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "tdse_last_run_ms": (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     "tdse_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "tdse_profile_read": (ctypes.c_int, [ctypes.c_void_p, c_double_p, c_int64_p, c_double_p, c_int64_p]),

@@ -97,6 +97,8 @@ struct ShHost {
     bool live = false;
     ShSim dev{};
     int n_data = 0;
+    int wave_first = 0;  // this simulation's range in the wave table
+    int wave_count = 0;
     std::vector<void*> blocks;
 };
 
@@ -195,8 +197,10 @@ int rebuild_tables(tdse_ctx* ctx) {
     std::vector<int2> waves, jobs;
     std::map<std::pair<int, int>, std::vector<int2>> shb, lnb;
     for (int s = 0; s < (int)ctx->sh.size(); ++s) {
-        const ShHost& h = ctx->sh[s];
+        ShHost& h = ctx->sh[s];
         if (!h.live) continue;
+        h.wave_first = (int)waves.size();
+        h.wave_count = h.dev.npad / TDSE_WAVE;
         for (int p0 = 0; p0 < h.dev.npad; p0 += TDSE_WAVE) waves.push_back(make_int2(s, p0));
         jobs.push_back(make_int2(s, -1));
         for (int b = 0; b < h.dev.n_basis; ++b) jobs.push_back(make_int2(s, b));
@@ -260,7 +264,7 @@ int launch_step(tdse_ctx* ctx) {
     if (nw > 0) {
         prof_begin(ctx, 0);
         hipLaunchKernelGGL(k_sh_lphase, dim3((nw * TDSE_WAVE + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh,
-                           ctx->waves.ptr, nw, ctx->d_cl);
+                           ctx->waves.ptr, nw, ctx->d_cl, 0);
         prof_end(ctx);
         const int nj = (int)ctx->jobs.count;
         hipLaunchKernelGGL(k_sh_observe, dim3(nj), dim3(256), 0, ctx->stream, ctx->d_sh, ctx->jobs.ptr, nj);
@@ -733,7 +737,10 @@ int tdse_upload_state(tdse_ctx* ctx, int32_t kind, int32_t slot, const double* g
         const ShSim& S = ctx->sh[slot].dev;
         if (step < 0 || step > S.N + 1) return fail(ctx, TDSE_ERR_ARG, "step out of range");
         HIP_TRY(ctx, hipMemcpy(S.g, g, (size_t)S.L * S.npad * sizeof(double2), hipMemcpyHostToDevice));
-        HIP_TRY(ctx, hipMemcpy(reinterpret_cast<char*>(ctx->d_sh + slot) + offsetof(ShSim, step), &step, sizeof(int),
+        const int st[2] = {step, 1};  // the uploaded g is the reference g after `step` steps
+        HIP_TRY(ctx, hipMemcpy(reinterpret_cast<char*>(ctx->d_sh + slot) + offsetof(ShSim, step), &st[0], sizeof(int),
+                               hipMemcpyHostToDevice));
+        HIP_TRY(ctx, hipMemcpy(reinterpret_cast<char*>(ctx->d_sh + slot) + offsetof(ShSim, phase), &st[1], sizeof(int),
                                hipMemcpyHostToDevice));
     } else if (kind == 1) {
         if (slot < 0 || slot >= (int)ctx->line.size() || !ctx->line[slot].live)
@@ -770,6 +777,35 @@ int tdse_remove(tdse_ctx* ctx, int32_t kind, int32_t slot) {
         return fail(ctx, TDSE_ERR_ARG, "unknown simulation kind");
     }
     ctx->dirty = true;
+    return TDSE_OK;
+}
+
+int tdse_materialize(tdse_ctx* ctx, int32_t kind, int32_t slot) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    if (kind == 1) {
+        if (slot < 0 || slot >= (int)ctx->line.size() || !ctx->line[slot].live)
+            return fail(ctx, TDSE_ERR_ARG, "bad Line slot");
+        return TDSE_OK;  // a LineMesh always holds the reference g between launches
+    }
+    if (kind != 0) return fail(ctx, TDSE_ERR_ARG, "unknown simulation kind");
+    if (slot < 0 || slot >= (int)ctx->sh.size() || !ctx->sh[slot].live) return fail(ctx, TDSE_ERR_ARG, "bad SH slot");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int rc;
+    if (ctx->dirty && (rc = rebuild_tables(ctx))) return rc;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    int st[2] = {0, 0};  // step, phase
+    char* base = reinterpret_cast<char*>(ctx->d_sh + slot);
+    HIP_TRY(ctx, hipMemcpy(&st[0], base + offsetof(ShSim, step), sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(&st[1], base + offsetof(ShSim, phase), sizeof(int), hipMemcpyDeviceToHost));
+    const ShHost& H = ctx->sh[slot];
+    if (st[1] == 1 || st[0] == 0 || st[0] > H.dev.N) return TDSE_OK;
+    const int nw = H.wave_count;
+    hipLaunchKernelGGL(k_sh_lphase, dim3((nw * TDSE_WAVE + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh,
+                       ctx->waves.ptr + H.wave_first, nw, ctx->d_cl, 1);
+    HIP_TRY(ctx, hipGetLastError());
+    const int one = 1;
+    HIP_TRY(ctx, hipMemcpyAsync(base + offsetof(ShSim, phase), &one, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return TDSE_OK;
 }
 

@@ -44,6 +44,7 @@ struct ShSim {
     int n, L, P, m, npad, N, store_every, LB, n_basis, nwaves;
     int step;              // steps already advanced; device-updated
     int active;
+    int phase;             // 0: g is mid-step (after the R-phase); 1: g is the reference g after `step` steps
 };
 
 enum { SH_T = 0, SH_U = 1, SH_AP = 2, SH_AN = 3, SH_RQ = 4, SH_MASK = 5, SH_NTAB = 6 };
@@ -82,8 +83,10 @@ __device__ __forceinline__ void sh_rotate(double a, double2& x0, double2& x1) {
 //   n == N      : end only       O_{N-1}, E_{N-1}, mask
 // At data times the wavefunction between "end" and "start" (the reference's
 // g after n steps) is observed: per-wave |g|^2 partials and the rows l < LB.
+// end_only != 0 materialises the reference g after n steps (O_{n-1}, E_{n-1},
+// mask) for the listed waves without starting step n and without observing.
 __global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, const int2* __restrict__ waves,
-                                                   int n_waves, const double* __restrict__ cl) {
+                                                   int n_waves, const double* __restrict__ cl, int end_only) {
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / TDSE_WAVE;
     const int lane = threadIdx.x % TDSE_WAVE;
     if (wave >= n_waves) return;
@@ -92,9 +95,10 @@ __global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, con
     const int n = S.step;
     const int N = S.N;
     if (!S.active || n > N) return;
-    const bool do_end = n > 0;
-    const bool do_start = n < N;
-    const bool observe = sh_is_data_step(n, N, S.store_every);
+    const bool do_end = n > 0 && S.phase == 0;
+    const bool do_start = n < N && !end_only;
+    const bool observe = !end_only && sh_is_data_step(n, N, S.store_every);
+    if (!do_end && !do_start && !observe) return;
 
     const int L = S.L, P = S.P, m = S.m, npad = S.npad;
     const int p = wt.y + lane;
@@ -273,5 +277,8 @@ __global__ void __launch_bounds__(MAXT) k_sh_rphase(ShSim* __restrict__ sims, co
 // Advance every active simulation's step counter after a full graph step.
 __global__ void k_sh_advance(ShSim* sims, int n_sims) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n_sims && sims[s].active && sims[s].step <= sims[s].N) sims[s].step += 1;
+    if (s < n_sims && sims[s].active && sims[s].step <= sims[s].N) {
+        sims[s].step += 1;
+        sims[s].phase = 0;
+    }
 }

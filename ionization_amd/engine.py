@@ -436,6 +436,10 @@ class DeviceContext:
         gp = np.ascontiguousarray(to_positions(np.asarray(g_natural, np.complex128), c.P, c.m))
         self._check(self._lib.tdse_upload_state(self._ctx, kind, slot, _lib.dptr(gp), int(step)))
 
+    def materialize(self, kind: int, slot: int):
+        """Make the device g equal to the reference g after the current step count."""
+        self._check(self._lib.tdse_materialize(self._ctx, kind, slot))
+
     def remove(self, kind: int, slot: int):
         self._check(self._lib.tdse_remove(self._ctx, kind, slot))
         (self._sh if kind == KIND_SH else self._line).pop(slot, None)

@@ -1,0 +1,20 @@
+"""Drop-in mirror of ``ionization.mesh`` whose evolve step runs on the GPU."""
+
+from .meshes import QuantumMesh, LineMesh, SphericalHarmonicMesh, CylindricalSliceMesh, WrappingDirection
+from .mesh_operators import (
+    MeshOperators,
+    KineticEnergyDerivation,
+    LineLengthGaugeOperators,
+    CylindricalSliceLengthGaugeOperators,
+    SphericalHarmonicLengthGaugeOperators,
+)
+from .evolution_methods import EvolutionMethod, SplitInteractionOperator, AlternatingDirectionImplicit
+from .sims import (
+    MeshSimulation,
+    MeshSpecification,
+    LineSpecification,
+    CylindricalSliceSpecification,
+    SphericalHarmonicSimulation,
+    SphericalHarmonicSpecification,
+)
+from .data import Data, Datastore, Fields, Norm, InnerProducts, DEFAULT_DATASTORE_TYPES

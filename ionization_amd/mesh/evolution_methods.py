@@ -1,0 +1,106 @@
+"""
+Evolution methods (``ionization/mesh/evolution_methods.py``) backed by the
+HIP kernels.
+
+Each method keeps the reference's strategy seam ``evolve(mesh, g, time_step)
+-> g`` and adds ``compile(sim)``, which turns a simulation into the device
+table form the engine runs. ``MeshSimulation.run`` uses ``compile`` and runs
+the whole time grid on the device; a bare ``evolve`` call does one reference
+step through the same kernels.
+"""
+
+import numpy as np
+
+from .. import core, engine
+from .. import units as u
+from . import meshes
+
+
+class EvolutionMethod:
+    """Strategy for advancing a mesh wavefunction (evolution_methods.py:16-43)."""
+
+    def evolve(self, mesh, g, time_step):
+        """One step of ``g`` by ``time_step`` on the device; returns the new host g."""
+        sim = mesh.sim
+        mesh.g = np.asarray(g, dtype=np.complex128)
+        sim._device_single_step(time_step)
+        return mesh.g
+
+    def compile(self, sim, g0=None):
+        raise NotImplementedError
+
+    def __repr__(self):
+        return f"{self.__class__.__name__}()"
+
+    def info(self):
+        return {"evolution_method": self.__class__.__name__}
+
+
+def _field(sim):
+    pot = sim.spec.electric_potential
+    return lambda t: np.asarray(pot.get_electric_field_amplitude(t), dtype=np.float64)
+
+
+def _mask(sim):
+    mask = sim.spec.mask
+    return lambda r: np.asarray(mask(r=r), dtype=np.float64)
+
+
+def _potential(sim):
+    pot = sim.spec.internal_potential
+    return lambda r: np.asarray(pot(r=r, test_charge=sim.spec.test_charge), dtype=np.float64)
+
+
+class SplitInteractionOperator(EvolutionMethod):
+    """
+    Split-operator step for a SphericalHarmonicMesh in the length gauge
+    (evolution_methods.py:80-123): E, O, CN explicit, CN implicit, O, E, then
+    the mask, with the 2x2 rotations on the l-flattened pairs.
+    """
+
+    def compile(self, sim, g0=None):
+        mesh = sim.mesh
+        if not isinstance(mesh, meshes.SphericalHarmonicMesh):
+            raise NotImplementedError("SplitInteractionOperator runs on SphericalHarmonicMesh only")
+        if sim.spec.operators.gauge != core.Gauge.LENGTH:
+            raise NotImplementedError("the device split operator implements the length gauge")
+        basis = []
+        for state in sim.spec.test_states:
+            comps = list(state)
+            if len(comps) != 1:
+                raise NotImplementedError("test states must have one spherical-harmonic component")
+            s = comps[0]
+            basis.append((np.real(mesh.get_radial_g_for_state(s)), s.l))
+        step = sim.spec.time_step
+        return engine.compile_sh(
+            r_bound=sim.spec.r_bound, r_points=sim.spec.r_points, l_bound=sim.spec.l_bound,
+            potential=_potential(sim), field=_field(sim), mask=_mask(sim), times=sim.times, time_step=step,
+            g0=mesh.g if g0 is None else g0, basis=basis, store_every=sim.spec.store_data_every,
+            test_charge=sim.spec.test_charge,
+            hydrogen_correction=sim.spec.operators.hydrogen_zero_angular_momentum_correction,
+            rows=getattr(sim.spec, "device_rows", None),
+        )
+
+
+class AlternatingDirectionImplicit(EvolutionMethod):
+    """
+    Crank-Nicolson / ADI step (evolution_methods.py:46-77). On a LineMesh the
+    total Hamiltonian is one operator, so a step is (1 + i tau H)^-1 (1 - i tau H)
+    followed by the mask.
+    """
+
+    def compile(self, sim, g0=None):
+        mesh = sim.mesh
+        if isinstance(mesh, meshes.LineMesh):
+            if sim.spec.operators.gauge != core.Gauge.LENGTH:
+                raise NotImplementedError("the device LineMesh step implements the length gauge")
+            basis = [np.asarray(mesh.get_g_for_state(s), dtype=np.complex128) for s in sim.spec.test_states]
+            return engine.compile_line(
+                z_bound=sim.spec.z_bound, z_points=sim.spec.z_points,
+                potential=lambda z: np.asarray(sim.spec.internal_potential(r=z, z=z, test_charge=sim.spec.test_charge),
+                                               dtype=np.float64),
+                field=_field(sim), mask=_mask(sim), times=sim.times, g0=mesh.g if g0 is None else g0, basis=basis,
+                store_every=sim.spec.store_data_every, test_mass=sim.spec.test_mass,
+                test_charge=sim.spec.test_charge, rows=getattr(sim.spec, "device_rows", None),
+            )
+        raise NotImplementedError(f"device ADI is not implemented for {mesh.__class__.__name__}")

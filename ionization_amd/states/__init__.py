@@ -116,6 +116,10 @@ class HydrogenBoundState(SphericalHarmonicState):
         lag_poly = special.eval_genlaguerre(n - l - 1, (2 * l) + 1, x)
         return self.amplitude * normalization * r_dep * lag_poly
 
+    def __call__(self, r, theta, phi):
+        """psi(r, theta, phi) = R_nl(r) Y_l^m(theta, phi) (three_d.py:132-205, 298-391)."""
+        return self.radial_function(r) * special.sph_harm_y(self.l, self.m, theta, phi)
+
     def __repr__(self):
         return f"HydrogenBoundState(n={self.n}, l={self.l}, m={self.m})"
 

@@ -1,0 +1,191 @@
+"""CPU tests of the host side: layouts, tables, the mesh API, the C-ABI library surface."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from ionization_amd import _lib, engine, exceptions, mesh, potentials, states, workloads
+from ionization_amd import units as u
+from oracle import mesh_cpu, physics
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "tdse_hip.h")).read()
+    return sorted(set(re.findall(r"\b(tdse_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = header_symbols()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(_lib.SIGNATURES), "ctypes signatures and header disagree"
+    assert _lib.load().tdse_abi_version() == 1
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "_LIB", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
+    with pytest.raises(exceptions.DeviceError):
+        _lib.load()
+
+
+def test_error_path_without_device():
+    """Creating a context with no visible GPU reports an error string instead of crashing."""
+    lib = _lib.load()
+    handle = ctypes.c_void_p()
+    rc = lib.tdse_ctx_create(0, 1 << 20, ctypes.byref(handle))
+    if rc == 0:  # a GPU is visible: nothing to check here
+        lib.tdse_ctx_destroy(handle)
+        pytest.skip("a HIP device is visible")
+    assert rc < 0 and lib.tdse_last_error(None)
+
+
+@pytest.mark.parametrize("n", [2, 63, 64, 200, 201, 500, 1000, 2049, 4000, 4096])
+def test_layout_roundtrip(n):
+    P, m = engine.choose_layout(n, "sh")
+    assert P % 64 == 0 and m % 2 == 0 and 2 <= m <= 16 and P * m >= n
+    assert P <= (512 if m <= 8 else 256)
+    j = engine.position_index(n, P, m)
+    assert np.array_equal(np.sort(j), np.arange(P * m))
+    a = np.random.default_rng(n).standard_normal((3, n)) + 1j
+    b = engine.from_positions(engine.to_positions(a, P, m), n, P, m)
+    assert np.array_equal(a, b)
+    # chunk k holds rows k m .. k m + m - 1: position i P + k
+    pos = engine.to_positions(np.arange(n, dtype=float)[None], P, m)[0]
+    for k in (0, P - 1):
+        for i in range(m):
+            jj = k * m + i
+            assert pos[i * P + k] == (jj if jj < n else 0)
+
+
+def test_line_layout_supports_8192():
+    assert engine.choose_layout(8192, "line") in [(1024, 8), (512, 16)]
+
+
+@pytest.mark.parametrize("N", [1, 2, 7, 100, 101])
+@pytest.mark.parametrize("every", [1, 3, 7, 100, -1])
+def test_data_cadence(N, every):
+    idx = engine.data_indices(N, every)
+    assert len(idx) == engine.data_count(N, every)
+    assert idx[0] == 0 and idx[-1] == N
+    # the device slot formula
+    for slot, n in enumerate(idx):
+        if every >= 1:
+            dev = N // every + 1 if (n == N and N % every) else n // every
+        else:
+            dev = 0 if n == 0 else 1
+        assert dev == slot
+
+
+def _small_sh_spec(**kw):
+    return mesh.SphericalHarmonicSpecification(
+        "t", r_bound=30 * u.bohr_radius, r_points=150, l_bound=12,
+        electric_potential=potentials.SincPulse(pulse_width=80 * u.asec, fluence=0.5 * u.Jcm2),
+        mask=potentials.RadialCosineMask(25 * u.bohr_radius, 30 * u.bohr_radius),
+        time_initial=-40 * u.asec, time_final=40 * u.asec, time_step=1 * u.asec, **kw)
+
+
+@pytest.mark.parametrize("n, l", [(1, 0), (2, 0), (2, 1)])
+def test_initial_wavefunction_is_normalized_with_numeric_eigenstates(n, l):
+    """tests/mesh/test_sims.py:13-28 of the reference."""
+    sim = mesh.SphericalHarmonicSpecification(
+        "test", initial_state=states.HydrogenBoundState(n, l), r_bound=50 * u.bohr_radius, r_points=250, l_bound=30,
+        use_numeric_eigenstates=True, numeric_eigenstate_max_energy=20 * u.eV,
+        numeric_eigenstate_max_angular_momentum=10).to_sim()
+    assert np.allclose(sim.mesh.norm(), 1)
+    assert sim.spec.initial_state.numeric and sim.spec.initial_state.l == l
+
+
+def test_numeric_basis_matches_arpack_set():
+    """Same eigenvalue set as the reference's eigsh loop (meshes.py:1290-1305) for one l."""
+    import scipy.sparse.linalg as sla
+
+    sim = _small_sh_spec().to_sim()
+    st = [s for s in sim.spec.test_states if s.l == 1]
+    d, e = sim.spec.operators.internal_hamiltonian_for_single_l(sim.mesh, 1)
+    import scipy.sparse as sparse
+
+    h = sparse.diags((e, d, e), offsets=(-1, 0, 1))
+    w, _ = sla.eigsh(h, k=len(st) + 3, which="SA")
+    w = np.sort(w)
+    kept = w[w <= sim.spec.numeric_eigenstate_max_energy]
+    assert np.allclose(sorted(s.energy for s in st), kept[: len(st)], rtol=1e-10)
+
+
+def test_device_tables_reproduce_oracle_hamiltonian():
+    """The compiled tables encode the oracle's sparse H0 and interaction couplings exactly (to rounding)."""
+    sim = _small_sh_spec().to_sim()
+    comp = sim.spec.evolution_method.compile(sim)
+    tau0 = sim.spec.time_step / (2 * u.hbar)
+    orc = mesh_cpu.SHOracle(r_bound=sim.spec.r_bound, r_points=150, l_bound=12, potential=physics.coulomb,
+                            field=lambda t: 0.0 * t)
+    n, L = 150, 12
+    tab = engine.from_positions(comp.tables, n, comp.P, comp.m)
+    T, U, AP, AN, RQ, MK = tab
+    h0 = orc.h0.tocsr()
+    for l in (0, 1, 5, 11):
+        blk = h0[l * n:(l + 1) * n, l * n:(l + 1) * n]
+        diag = (T + l * (l + 1) * U) / tau0
+        if l == 0:
+            diag = diag.copy()
+            diag[0] += comp.corr0 / tau0
+        assert np.allclose(diag, blk.diagonal().real, rtol=1e-13, atol=0)
+        assert np.allclose(AN[:-1] / tau0, blk.diagonal(1).real, rtol=1e-13, atol=0)
+        assert np.allclose(AP[1:] / tau0, blk.diagonal(-1).real, rtol=1e-13, atol=0)
+    assert AN[-1] == 0 and AP[0] == 0
+    # interaction coupling of pair (l, l+1) at r_j: RQ_j c_l / tau0 == oracle's l_interaction entry
+    li = orc.l_interaction.tocsr()
+    for j in (0, 7, 149):
+        for l in (0, 3, 10):
+            k = j * L + l
+            cl = (l + 1) / np.sqrt((2 * l + 1) * (2 * l + 3))
+            assert np.isclose(RQ[j] * cl / tau0, li[k + 1, k].real, rtol=1e-13)
+    assert np.allclose(MK, sim.spec.mask(r=sim.mesh.r))
+
+
+def test_field_table_samples_t_next_plus_half_step():
+    sim = _small_sh_spec().to_sim()
+    comp = sim.spec.evolution_method.compile(sim)
+    t = sim.times
+    expect = sim.spec.electric_potential.get_electric_field_amplitude(t[1:] + sim.spec.time_step / 2)
+    assert np.allclose(comp.fs, expect * np.diff(t) / sim.spec.time_step, rtol=1e-15, atol=0)
+
+
+def test_data_access_errors():
+    sim = _small_sh_spec(datastores=[mesh.Norm()]).to_sim()
+    with pytest.raises(exceptions.MissingDatastore):
+        sim.data.bound_state_overlap
+    with pytest.raises(exceptions.UnknownData):
+        sim.data.definitely_not_data
+    with pytest.raises(exceptions.DuplicateDatastores):
+        _small_sh_spec(datastores=[mesh.Norm(), mesh.Norm()])
+
+
+def test_scan_draws_are_seeded_and_in_range():
+    a = workloads.sh_scan_params(256, 0)
+    b = workloads.sh_scan_params(256, 0)
+    assert a == b
+    for p in a:
+        assert 500 <= p["r_points"] <= 4000 and 50 <= p["l_bound"] <= 300
+        assert 50 * u.asec <= p["pulse_width"] <= 800 * u.asec
+        assert 0.01 * u.Jcm2 <= p["fluence"] <= 10 * u.Jcm2 and 0 <= p["phase"] <= np.pi
+
+
+def test_fixed_step_count():
+    spec = mesh.LineSpecification("l", initial_state=states.NumericOneDState(g=np.ones(64), energy=0.0),
+                                  z_bound=10 * u.bohr_radius, z_points=64, time_step=0.0413 * u.atomic_time,
+                                  time_final=4000 * 0.0413 * u.atomic_time)
+    workloads.fixed_step_count(spec, 4000)
+    assert int((spec.time_final - spec.time_initial) / spec.time_step) == 4000
+
+
+def test_hamiltonian_derivation_is_rejected():
+    with pytest.raises(NotImplementedError):
+        mesh.SphericalHarmonicLengthGaugeOperators(kinetic_energy_derivation="hamiltonian")
