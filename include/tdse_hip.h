@@ -109,7 +109,7 @@ int tdse_sh_add(tdse_ctx *ctx, const tdse_sh_desc *desc, const double *tables, c
  */
 typedef struct {
     int32_t z_points;      /* n */
-    int32_t chunks;        /* P, multiple of 64, <= 1024 */
+    int32_t chunks;        /* P, multiple of 64, <= 512 */
     int32_t rows;          /* m, 2 <= m <= 16 */
     int32_t n_steps;
     int32_t store_every;

@@ -4,7 +4,27 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "fastmath.h"
+
 #define TDSE_WAVE 64
+
+// Pointers read from descriptor structs are generic (flat) to the compiler;
+// these helpers access them through the global address space so the loads
+// and stores compile to global_load / global_store.
+typedef double tdse_d2v __attribute__((ext_vector_type(2)));
+#define TDSE_GLOBAL __attribute__((address_space(1)))
+
+__device__ __forceinline__ double2 ld2(const double2* p, size_t i) {
+    const tdse_d2v v = ((const TDSE_GLOBAL tdse_d2v*)p)[i];
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st2(double2* p, size_t i, double2 v) {
+    tdse_d2v w;
+    w.x = v.x;
+    w.y = v.y;
+    ((TDSE_GLOBAL tdse_d2v*)p)[i] = w;
+}
+__device__ __forceinline__ double ld1(const double* p, size_t i) { return ((const TDSE_GLOBAL double*)p)[i]; }
 
 // complex128 as double2 {re, im}
 __device__ __forceinline__ double2 cmk(double re, double im) { return make_double2(re, im); }
@@ -21,7 +41,7 @@ __device__ __forceinline__ double cabs2(double2 a) { return fma(a.x, a.x, a.y * 
 
 // 1 / z for complex z: conj(z) / |z|^2
 __device__ __forceinline__ double2 crecip(double2 z) {
-    double inv = 1.0 / fma(z.x, z.x, z.y * z.y);
+    const double inv = tdse_rcp(fma(z.x, z.x, z.y * z.y));
     return cmk(z.x * inv, -z.y * inv);
 }
 __device__ __forceinline__ double2 cdiv(double2 a, double2 b) { return cmul(a, crecip(b)); }

@@ -104,6 +104,7 @@ struct ShHost {
 
 struct LineHost {
     bool live = false;
+    bool complex_steps = false;  // some w_n has a nonzero real part
     LineSim dev{};
     int n_data = 0;
     std::vector<void*> blocks;
@@ -118,10 +119,15 @@ struct DevVec {
 };
 
 struct Bucket {
-    int M = 0;
+    int M = 0;  // exact rows per chunk-thread
     int P = 0;
+    bool imag = false;  // Line: complex time steps (imaginary-time relaxation)
     DevVec<int2> lines;  // SH: (slot, l); Line: (slot, 0)
 };
+
+// l segments of at most kSegRows rows per L-phase wave, balanced in size.
+constexpr int kSegRows = 64;
+int sh_segments(int L) { return (L + kSegRows - 1) / kSegRows; }
 
 int n_data_points(int N, int every) {
     if (every >= 1) return N / every + 1 + ((N % every) != 0 ? 1 : 0);
@@ -140,7 +146,7 @@ struct tdse_ctx {
     ShSim* d_sh = nullptr;
     LineSim* d_line = nullptr;
     double* d_cl = nullptr;
-    DevVec<int2> waves;
+    DevVec<ShWave> waves;
     DevVec<int2> jobs;
     std::vector<Bucket> sh_buckets;
     std::vector<Bucket> line_buckets;
@@ -194,25 +200,30 @@ void drop_graph(tdse_ctx* ctx) {
 
 // Rebuild the ragged work tables from the live simulations.
 int rebuild_tables(tdse_ctx* ctx) {
-    std::vector<int2> waves, jobs;
-    std::map<std::pair<int, int>, std::vector<int2>> shb, lnb;
+    std::vector<ShWave> waves;
+    std::vector<int2> jobs;
+    std::map<std::pair<int, int>, std::vector<int2>> shb, lnb;  // (rows, chunks [+ 4096 if complex steps])
     for (int s = 0; s < (int)ctx->sh.size(); ++s) {
         ShHost& h = ctx->sh[s];
         if (!h.live) continue;
         h.wave_first = (int)waves.size();
-        h.wave_count = h.dev.npad / TDSE_WAVE;
-        for (int p0 = 0; p0 < h.dev.npad; p0 += TDSE_WAVE) waves.push_back(make_int2(s, p0));
+        const int nseg = sh_segments(h.dev.L);
+        const int seg = (h.dev.L + nseg - 1) / nseg;
+        const int wpl = h.dev.npad / TDSE_WAVE;
+        for (int sg = 0; sg < nseg; ++sg) {
+            const int lo = sg * seg, hi = std::min(h.dev.L, lo + seg);
+            for (int w = 0; w < wpl; ++w) waves.push_back(ShWave{s, w * TDSE_WAVE, lo, hi, sg * wpl + w});
+        }
+        h.wave_count = (int)waves.size() - h.wave_first;
         jobs.push_back(make_int2(s, -1));
         for (int b = 0; b < h.dev.n_basis; ++b) jobs.push_back(make_int2(s, b));
-        const int M = h.dev.m <= 8 ? 8 : 16;
-        auto& v = shb[{M, h.dev.P}];
+        auto& v = shb[{h.dev.m, h.dev.P}];
         for (int l = 0; l < h.dev.L; ++l) v.push_back(make_int2(s, l));
     }
     for (int s = 0; s < (int)ctx->line.size(); ++s) {
         const LineHost& h = ctx->line[s];
         if (!h.live) continue;
-        const int M = h.dev.m <= 8 ? 8 : 16;
-        lnb[{M, h.dev.P}].push_back(make_int2(s, 0));
+        lnb[{h.dev.m, h.dev.P + (h.complex_steps ? 4096 : 0)}].push_back(make_int2(s, 0));
     }
     int rc;
     if ((rc = devvec_upload(ctx, ctx->waves, waves))) return rc;
@@ -224,7 +235,8 @@ int rebuild_tables(tdse_ctx* ctx) {
         for (auto& kv : src) {
             Bucket b;
             b.M = kv.first.first;
-            b.P = kv.first.second;
+            b.P = kv.first.second % 4096;
+            b.imag = kv.first.second >= 4096;
             int r = devvec_upload(ctx, b.lines, kv.second);
             if (r) return r;
             dst.push_back(b);
@@ -258,6 +270,52 @@ void prof_end(tdse_ctx* ctx) {
     (void)hipEventRecord(next_event(ctx), ctx->stream);
 }
 
+template <int M, int MAXT>
+void launch_rphase(tdse_ctx* ctx, const Bucket& b) {
+    const size_t lds = 3 * (size_t)b.P * sizeof(double2);
+    hipLaunchKernelGGL((k_sh_rphase<M, MAXT>), dim3((unsigned)b.lines.count), dim3(b.P), lds, ctx->stream, ctx->d_sh,
+                       b.lines.ptr, (int)b.lines.count);
+}
+
+void dispatch_rphase(tdse_ctx* ctx, const Bucket& b) {
+    switch (b.M) {
+        case 2: launch_rphase<2, 512>(ctx, b); break;
+        case 4: launch_rphase<4, 512>(ctx, b); break;
+        case 6: launch_rphase<6, 512>(ctx, b); break;
+        case 8: launch_rphase<8, 512>(ctx, b); break;
+        case 10: launch_rphase<10, 256>(ctx, b); break;
+        case 12: launch_rphase<12, 256>(ctx, b); break;
+        case 14: launch_rphase<14, 256>(ctx, b); break;
+        default: launch_rphase<16, 256>(ctx, b); break;
+    }
+}
+
+template <int M, bool IMAG>
+void launch_line(tdse_ctx* ctx, const Bucket& b, int n_steps) {
+    const size_t lds = 3 * (size_t)b.P * sizeof(double2) + 64 * sizeof(double);
+    hipLaunchKernelGGL((k_line_steps<M, 512, IMAG>), dim3((unsigned)b.lines.count), dim3(b.P), lds, ctx->stream,
+                       ctx->d_line, b.lines.ptr, (int)b.lines.count, n_steps);
+}
+
+template <bool IMAG>
+void dispatch_line_m(tdse_ctx* ctx, const Bucket& b, int n_steps) {
+    switch (b.M) {
+        case 2: launch_line<2, IMAG>(ctx, b, n_steps); break;
+        case 4: launch_line<4, IMAG>(ctx, b, n_steps); break;
+        case 6: launch_line<6, IMAG>(ctx, b, n_steps); break;
+        case 8: launch_line<8, IMAG>(ctx, b, n_steps); break;
+        case 10: launch_line<10, IMAG>(ctx, b, n_steps); break;
+        case 12: launch_line<12, IMAG>(ctx, b, n_steps); break;
+        case 14: launch_line<14, IMAG>(ctx, b, n_steps); break;
+        default: launch_line<16, IMAG>(ctx, b, n_steps); break;
+    }
+}
+
+void dispatch_line(tdse_ctx* ctx, const Bucket& b, int n_steps) {
+    if (b.imag) dispatch_line_m<false>(ctx, b, n_steps);
+    else dispatch_line_m<true>(ctx, b, n_steps);
+}
+
 // Launch one full step of every simulation kind onto the context stream.
 int launch_step(tdse_ctx* ctx) {
     const int nw = (int)ctx->waves.count;
@@ -269,30 +327,14 @@ int launch_step(tdse_ctx* ctx) {
         const int nj = (int)ctx->jobs.count;
         hipLaunchKernelGGL(k_sh_observe, dim3(nj), dim3(256), 0, ctx->stream, ctx->d_sh, ctx->jobs.ptr, nj);
         for (auto& b : ctx->sh_buckets) {
-            const size_t lds = 3 * (size_t)b.P * sizeof(double2);
-            const int nl = (int)b.lines.count;
             prof_begin(ctx, 1);
-            if (b.M == 8)
-                hipLaunchKernelGGL((k_sh_rphase<8, 512>), dim3(nl), dim3(b.P), lds, ctx->stream, ctx->d_sh,
-                                   b.lines.ptr, nl);
-            else
-                hipLaunchKernelGGL((k_sh_rphase<16, 256>), dim3(nl), dim3(b.P), lds, ctx->stream, ctx->d_sh,
-                                   b.lines.ptr, nl);
+            dispatch_rphase(ctx, b);
             prof_end(ctx);
         }
         const int ns = (int)ctx->sh.size();
         hipLaunchKernelGGL(k_sh_advance, dim3((ns + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh, ns);
     }
-    for (auto& b : ctx->line_buckets) {
-        const size_t lds = 3 * (size_t)b.P * sizeof(double2) + 64 * sizeof(double);
-        const int nl = (int)b.lines.count;
-        if (b.M == 8)
-            hipLaunchKernelGGL((k_line_steps<8, 1024>), dim3(nl), dim3(b.P), lds, ctx->stream, ctx->d_line,
-                               b.lines.ptr, nl, 1);
-        else
-            hipLaunchKernelGGL((k_line_steps<16, 512>), dim3(nl), dim3(b.P), lds, ctx->stream, ctx->d_line,
-                               b.lines.ptr, nl, 1);
-    }
+    for (auto& b : ctx->line_buckets) dispatch_line(ctx, b, 1);
     HIP_TRY(ctx, hipGetLastError());
     return TDSE_OK;
 }
@@ -502,7 +544,7 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
     S.store_every = D.store_every;
     S.LB = D.lb;
     S.n_basis = D.n_basis;
-    S.nwaves = S.npad / TDSE_WAVE;
+    S.nwaves = (S.npad / TDSE_WAVE) * sh_segments(S.L);
     S.corr0 = D.corr0;
     S.delta_r = D.delta_r;
     S.step = 0;
@@ -555,8 +597,8 @@ int tdse_line_add(tdse_ctx* ctx, const tdse_line_desc* desc, const double* table
     const int P = D.chunks, m = D.rows;
     if (D.z_points < 2) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: z_points must be >= 2");
     if (P < TDSE_WAVE || P % TDSE_WAVE) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: chunks must be a positive multiple of 64");
-    if (m < 2 || m > 16) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: rows must be in [2, 16]");
-    if ((m <= 8 && P > 1024) || (m > 8 && P > 512)) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: too many chunks for rows");
+    if (m < 2 || m > 16 || (m & 1)) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: rows must be even and in [2, 16]");
+    if (P > 512) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: chunks must be <= 512");
     if ((int64_t)P * m < D.z_points) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: chunks * rows < z_points");
     if (D.n_steps < 1) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: n_steps must be >= 1");
     if (D.store_every == 0 || D.store_every < -1) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: store_every must be >= 1 or -1");
@@ -575,6 +617,9 @@ int tdse_line_add(tdse_ctx* ctx, const tdse_line_desc* desc, const double* table
     }
     LineHost& H = ctx->line[s];
     H.blocks.clear();
+    H.complex_steps = false;
+    for (int i = 0; i < D.n_steps; ++i)
+        if (wt[2 * i] != 0.0) H.complex_steps = true;
     LineSim& S = H.dev;
     S = LineSim{};
     S.n = D.z_points;
@@ -658,16 +703,7 @@ int tdse_line_run(tdse_ctx* ctx, int32_t n_steps) {
     int rc;
     if (ctx->dirty && (rc = rebuild_tables(ctx))) return rc;
     HIP_TRY(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
-    for (auto& b : ctx->line_buckets) {
-        const size_t lds = 3 * (size_t)b.P * sizeof(double2) + 64 * sizeof(double);
-        const int nl = (int)b.lines.count;
-        if (b.M == 8)
-            hipLaunchKernelGGL((k_line_steps<8, 1024>), dim3(nl), dim3(b.P), lds, ctx->stream, ctx->d_line,
-                               b.lines.ptr, nl, n_steps);
-        else
-            hipLaunchKernelGGL((k_line_steps<16, 512>), dim3(nl), dim3(b.P), lds, ctx->stream, ctx->d_line,
-                               b.lines.ptr, nl, n_steps);
-    }
+    for (auto& b : ctx->line_buckets) dispatch_line(ctx, b, n_steps);
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
     HIP_TRY(ctx, hipEventSynchronize(ctx->ev_stop));

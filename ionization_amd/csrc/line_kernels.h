@@ -55,33 +55,33 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 }
 
 template <int M>
-__device__ __forceinline__ void line_observe(const LineSim& S, const double2 (&x)[M], int m, int k, int P,
-                                             int slot, double* red) {
+__device__ __forceinline__ void line_observe(const LineSim& S, const double2 (&x)[M], int k, int P, int slot,
+                                             double* red) {
     double acc = 0.0;
 #pragma unroll
-    for (int i = 0; i < M; ++i)
-        if (i < m) acc += cabs2(x[i]);
+    for (int i = 0; i < M; ++i) acc += cabs2(x[i]);
     const double nrm = block_sum(acc, red);
     if (threadIdx.x == 0) S.out_norm[slot] = nrm * S.delta_z;
     for (int b = 0; b < S.n_basis; ++b) {
         const double2* phi = S.basis + (size_t)b * S.npad + k;
         double re = 0.0, im = 0.0;
 #pragma unroll
-        for (int i = 0; i < M; ++i)
-            if (i < m) {
-                const double2 f = phi[i * P];
-                // conj(f) * x
-                re += f.x * x[i].x + f.y * x[i].y;
-                im += f.x * x[i].y - f.y * x[i].x;
-            }
+        for (int i = 0; i < M; ++i) {
+            const double2 f = ld2(phi, i * P);
+            // conj(f) * x
+            re += f.x * x[i].x + f.y * x[i].y;
+            im += f.x * x[i].y - f.y * x[i].x;
+        }
         const double sre = block_sum(re, red);
         const double sim = block_sum(im, red);
         if (threadIdx.x == 0) S.out_ip[(size_t)slot * S.n_basis + b] = cmk(sre * S.delta_z, sim * S.delta_z);
     }
 }
 
-// blockDim.x == P of every simulation launched together; one block per simulation.
-template <int M, int MAXT>
+// blockDim.x == P and m == M for every simulation launched together; one block per simulation.
+// IMAG: every w_n is purely imaginary (real time steps), which halves the
+// coefficient arithmetic; imaginary-time relaxation uses the general path.
+template <int M, int MAXT, bool IMAG>
 __global__ void __launch_bounds__(MAXT) k_line_steps(LineSim* __restrict__ sims, const int2* __restrict__ ids,
                                                      int n_ids, int n_steps) {
     extern __shared__ double2 lds[];  // 3 * P double2, then reduction scratch
@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(MAXT) k_line_steps(LineSim* __restrict__ sims,
     int n = S.step;
     const int N = S.N;
     if (n > N) return;
-    const int P = S.P, m = S.m, npad = S.npad;
+    const int P = S.P, npad = S.npad;
     const int k = threadIdx.x;
     double* red = reinterpret_cast<double*>(lds + 3 * P);
     double2* __restrict__ g = S.g + k;
@@ -103,46 +103,37 @@ __global__ void __launch_bounds__(MAXT) k_line_steps(LineSim* __restrict__ sims,
 
     double2 x[M];
 #pragma unroll
-    for (int i = 0; i < M; ++i)
-        if (i < m) x[i] = g[(size_t)i * P];
+    for (int i = 0; i < M; ++i) x[i] = ld2(g, (size_t)i * P);
 
-    if (n == 0 && line_is_data_step(0, N, S.store_every)) line_observe<M>(S, x, m, k, P, 0, red);
+    if (n == 0 && line_is_data_step(0, N, S.store_every)) line_observe<M>(S, x, k, P, 0, red);
 
     for (int it = 0; it < n_steps && n < N; ++it, ++n) {
-        const double2 w = S.wt[n];
-        const double f = S.fs[n];
-        double2 xlast = x[0];
-#pragma unroll
-        for (int i = 1; i < M; ++i)
-            if (i < m) xlast = x[i];
+        const double2 w = ld2(S.wt, n);
+        const double f = ld1(S.fs, n);
         double2 xl, xr;
-        chunk_neighbours<M>(x, xlast, k, P, lds, xl, xr, true);
+        chunk_neighbours<M>(x, k, P, lds, xl, xr, true);
         auto coef = [&](int i, double& hm, double& h, double& hp) {
-            h = fma(f, ZQ[i * P], H[i * P]);
-            hm = AP[i * P];
-            hp = AN[i * P];
+            h = fma(f, ld1(ZQ, i * P), ld1(H, i * P));
+            hm = ld1(AP, i * P);
+            hp = ld1(AN, i * P);
         };
-        cn_partition<M, false>(x, m, k, P, xl, xr, w, coef, lds, true);
+        cn_partition<M, IMAG>(x, k, P, xl, xr, w, coef, lds, true);
 #pragma unroll
-        for (int i = 0; i < M; ++i)
-            if (i < m) x[i] = cscale(MK[i * P], x[i]);
+        for (int i = 0; i < M; ++i) x[i] = cscale(ld1(MK, i * P), x[i]);
         if (S.renormalize) {
             // imaginary-time relaxation: restore unit norm after every step
             double acc = 0.0;
 #pragma unroll
-            for (int i = 0; i < M; ++i)
-                if (i < m) acc += cabs2(x[i]);
+            for (int i = 0; i < M; ++i) acc += cabs2(x[i]);
             const double inv = 1.0 / sqrt(block_sum(acc, red) * S.delta_z);
 #pragma unroll
-            for (int i = 0; i < M; ++i)
-                if (i < m) x[i] = cscale(inv, x[i]);
+            for (int i = 0; i < M; ++i) x[i] = cscale(inv, x[i]);
         }
         if (line_is_data_step(n + 1, N, S.store_every))
-            line_observe<M>(S, x, m, k, P, line_data_slot(n + 1, N, S.store_every), red);
+            line_observe<M>(S, x, k, P, line_data_slot(n + 1, N, S.store_every), red);
     }
 #pragma unroll
-    for (int i = 0; i < M; ++i)
-        if (i < m) g[(size_t)i * P] = x[i];
+    for (int i = 0; i < M; ++i) st2(g, (size_t)i * P, x[i]);
     __syncthreads();
     if (k == 0) S.step = (n == N) ? N + 1 : n;
 }

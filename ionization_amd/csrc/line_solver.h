@@ -18,7 +18,8 @@
 //    LDS with every equation normalised to a unit diagonal.
 //  * Each thread back-substitutes its rows.
 //
-// Requirements: 2 <= m <= M, all P threads of the line call this together
+// M is the exact number of rows per chunk (a compile-time constant, 2 <= M).
+// Requirements: all P threads of the line call this together
 // (block-wide barriers), lds holds 3 P double2 for this line. Rows beyond the
 // physical line are padding rows with H = 0 and x = 0 (they stay 0).
 #pragma once
@@ -33,34 +34,62 @@ __device__ __forceinline__ double2 wmul(double2 w, double2 v) {
 
 // coef(i, hm, h, hp): H_{j,j-1}, H_{j,j}, H_{j,j+1} of the thread's row i.
 // xl / xr: x_{km-1} and x_{km+m} (zero at the line ends).
-template <int M, bool IMAG, class Coef>
-__device__ __forceinline__ void cn_partition(double2 (&x)[M], const int m, const int k, const int P,
+// Off-diagonal coefficients w*hm, w*hp: complex in general; for IMAG (w = i s)
+// they are i times a real number and are kept as that real number.
+template <bool IMAG>
+struct OffDiag {
+    double2 v;
+    __device__ __forceinline__ static OffDiag make(double2 w, double h) {
+        OffDiag o;
+        o.v = cmk(w.x * h, w.y * h);
+        return o;
+    }
+    __device__ __forceinline__ double2 mul(double2 a) const { return cmul(v, a); }
+};
+template <>
+struct OffDiag<true> {
+    double s;  // the coefficient is i * s
+    __device__ __forceinline__ static OffDiag make(double2 w, double h) {
+        OffDiag o;
+        o.s = w.y * h;
+        return o;
+    }
+    __device__ __forceinline__ double2 mul(double2 a) const { return cimul(s, a); }
+};
+
+// RHS_IS_X: the right-hand side is x itself (implicit solve only, no explicit
+// half); xl / xr are then unused.
+template <int M, bool IMAG, class Coef, bool RHS_IS_X = false>
+__device__ __forceinline__ void cn_partition(double2 (&x)[M], const int k, const int P,
                                              const double2 xl, const double2 xr, const double2 w,
                                              Coef coef, double2* __restrict__ lds, const bool sync_first) {
+    using Off = OffDiag<IMAG>;
     double2 al[M - 1], ga[M - 1], de[M - 1];
     double2 alm = cmk(0.0, 0.0), gam = cmk(0.0, 0.0), dem = cmk(0.0, 0.0);  // row m-2
-    double2 dlast = cmk(0.0, 0.0), diaglast = cmk(1.0, 0.0), sublast = cmk(0.0, 0.0), suplast = cmk(0.0, 0.0);
+    double2 dlast = cmk(0.0, 0.0), diaglast = cmk(1.0, 0.0);
+    Off sublast = Off::make(w, 0.0), suplast = Off::make(w, 0.0);
 #pragma unroll
     for (int i = 0; i < M; ++i) {
-        if (i < m) {
+        {
             double hm, h, hp;
             coef(i, hm, h, hp);
-            const double2 xm = (i == 0) ? xl : x[i > 0 ? i - 1 : 0];
-            double2 xp = xr;
-#pragma unroll
-            for (int t = i + 1; t < M; ++t)
-                if (t == i + 1 && t < m) xp = x[t];
-            // explicit half: d = x - w (hm x_{j-1} + h x_j + hp x_{j+1})
-            const double2 hx = cmk(fma(hm, xm.x, fma(h, x[i].x, hp * xp.x)), fma(hm, xm.y, fma(h, x[i].y, hp * xp.y)));
-            const double2 d = csub(x[i], wmul<IMAG>(w, hx));
+            double2 d = x[i];
+            if constexpr (!RHS_IS_X) {
+                const double2 xm = (i == 0) ? xl : x[i > 0 ? i - 1 : 0];
+                const double2 xp = (i + 1 < M) ? x[i + 1 < M ? i + 1 : i] : xr;
+                // explicit half: d = x - w (hm x_{j-1} + h x_j + hp x_{j+1})
+                const double2 hx =
+                    cmk(fma(hm, xm.x, fma(h, x[i].x, hp * xp.x)), fma(hm, xm.y, fma(h, x[i].y, hp * xp.y)));
+                d = csub(x[i], wmul<IMAG>(w, hx));
+            }
             // implicit half coefficients: sub = w hm, diag = 1 + w h, sup = w hp
-            const double2 sub = wmul<IMAG>(w, cmk(hm, 0.0));
-            const double2 sup = wmul<IMAG>(w, cmk(hp, 0.0));
+            const Off sub = Off::make(w, hm);
+            const Off sup = Off::make(w, hp);
             const double2 diag = cadd(cmk(1.0, 0.0), wmul<IMAG>(w, cmk(h, 0.0)));
-            if (i < m - 1 && i < M - 1) {
-                const double2 den = (i == 0) ? diag : csub(diag, cmul(sub, ga[i > 0 ? i - 1 : 0]));
+            if (i < M - 1) {
+                const double2 den = (i == 0) ? diag : csub(diag, sub.mul(ga[i > 0 ? i - 1 : 0]));
                 const double2 r = crecip(den);
-                const double2 rsub = cmul(r, sub);
+                const double2 rsub = sub.mul(r);
                 if (i == 0) {
                     al[0] = rsub;
                     de[0] = cmul(r, d);
@@ -68,7 +97,7 @@ __device__ __forceinline__ void cn_partition(double2 (&x)[M], const int m, const
                     al[i] = cneg(cmul(rsub, al[i - 1]));
                     de[i] = csub(cmul(r, d), cmul(rsub, de[i - 1]));
                 }
-                ga[i] = cmul(r, sup);
+                ga[i] = sup.mul(r);
                 alm = al[i];
                 gam = ga[i];
                 dem = de[i];
@@ -84,11 +113,9 @@ __device__ __forceinline__ void cn_partition(double2 (&x)[M], const int m, const
     double2 ap0 = alm, gp0 = gam, dp0 = dem;
 #pragma unroll
     for (int i = M - 3; i >= 0; --i) {
-        if (i < m - 2) {
-            ap0 = csub(al[i], cmul(ga[i], ap0));
-            gp0 = cneg(cmul(ga[i], gp0));
-            dp0 = csub(de[i], cmul(ga[i], dp0));
-        }
+        ap0 = csub(al[i], cmul(ga[i], ap0));
+        gp0 = cneg(cmul(ga[i], gp0));
+        dp0 = csub(de[i], cmul(ga[i], dp0));
     }
     if (sync_first) __syncthreads();
     lds[k] = ap0;
@@ -102,10 +129,10 @@ __device__ __forceinline__ void cn_partition(double2 (&x)[M], const int m, const
         nd = lds[2 * P + k + 1];
     }
     // reduced equation of the chunk's last row: A Y_{k-1} + B Y_k + C Y_{k+1} = D
-    double2 A = cneg(cmul(sublast, alm));
-    const double2 B = csub(csub(diaglast, cmul(sublast, gam)), cmul(suplast, na));
-    double2 C = cneg(cmul(suplast, ng));
-    double2 D = csub(csub(dlast, cmul(sublast, dem)), cmul(suplast, nd));
+    double2 A = cneg(sublast.mul(alm));
+    const double2 B = csub(csub(diaglast, sublast.mul(gam)), suplast.mul(na));
+    double2 C = cneg(suplast.mul(ng));
+    double2 D = csub(csub(dlast, sublast.mul(dem)), suplast.mul(nd));
     {
         const double2 ib = crecip(B);
         A = cmul(A, ib);
@@ -149,22 +176,32 @@ __device__ __forceinline__ void cn_partition(double2 (&x)[M], const int m, const
     double2 y = D;
 #pragma unroll
     for (int i = M - 1; i >= 0; --i) {
-        if (i < m) {
-            if (i < m - 1 && i < M - 1) y = csub(csub(de[i], cmul(al[i], Lk)), cmul(ga[i], y));
-            x[i] = y;
-        }
+        if (i < M - 1) y = csub(csub(de[i], cmul(al[i], Lk)), cmul(ga[i], y));
+        x[i] = y;
     }
 }
 
+// Real-time form through the Cayley identity
+//     (1 + i H)^-1 (1 - i H) x = 2 (1 + i H)^-1 x - x,
+// which needs no explicit stencil (and no neighbour exchange): solve
+// (1 + i H) z = x in place with cn_partition's elimination (d = x); the
+// caller forms y = 2 z - x, re-reading x from memory (a cache hit) rather than
+// keeping a second copy live in registers.
+template <int M, class Coef>
+__device__ __forceinline__ void cn_cayley_solve(double2 (&x)[M], const int k, const int P, Coef coef,
+                                                double2* __restrict__ lds) {
+    cn_partition<M, true, Coef, true>(x, k, P, cmk(0.0, 0.0), cmk(0.0, 0.0), cmk(0.0, 1.0), coef, lds, false);
+}
+
 // Exchange the first / last rows of every chunk through LDS so each thread
-// knows x_{km-1} and x_{km+m}; xlast is the chunk's row m-1. lds: 2 P double2.
+// knows x_{km-1} and x_{km+m}. lds: 2 P double2.
 template <int M>
-__device__ __forceinline__ void chunk_neighbours(const double2 (&x)[M], const double2 xlast, const int k,
-                                                 const int P, double2* __restrict__ lds, double2& xl, double2& xr,
+__device__ __forceinline__ void chunk_neighbours(const double2 (&x)[M], const int k, const int P,
+                                                 double2* __restrict__ lds, double2& xl, double2& xr,
                                                  const bool sync_first) {
     if (sync_first) __syncthreads();
     lds[k] = x[0];
-    lds[P + k] = xlast;
+    lds[P + k] = x[M - 1];
     __syncthreads();
     xl = (k > 0) ? lds[P + k - 1] : cmk(0.0, 0.0);
     xr = (k + 1 < P) ? lds[k + 1] : cmk(0.0, 0.0);

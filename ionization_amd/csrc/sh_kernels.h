@@ -33,7 +33,7 @@ struct ShSim {
     const double* fs;      // [N] field times tau ratio per step: E(t_{n+1}+dt/2) * tau_n / tau0
     const double* ts;      // [N] tau_n / tau0
     double2* snap;         // [LB][npad] rows l < LB captured at data steps
-    double* part;          // [2][nwaves] per-wave partial |g|^2 sums (all l, largest l)
+    double* part;          // [2][nwaves] per-wave partial |g|^2 sums (all l, largest l); nwaves counts L-phase waves
     double* out_norm;      // [n_data]
     double* out_lnorm;     // [n_data]
     double2* out_ip;       // [n_data][n_basis]
@@ -62,36 +62,46 @@ __device__ __forceinline__ int sh_data_slot(int n, int N, int every) {
 // exp(-i a sigma_x) applied to (x0, x1):  y0 = cos a x0 - i sin a x1,  y1 = -i sin a x0 + cos a x1
 __device__ __forceinline__ void sh_rotate(double a, double2& x0, double2& x1) {
     double s, c;
-    sincos(a, &s, &c);
-    double2 y0 = cmk(fma(c, x0.x, s * x1.y), fma(c, x0.y, -s * x1.x));
-    double2 y1 = cmk(fma(c, x1.x, s * x0.y), fma(c, x1.y, -s * x0.x));
+    tdse_sincos(a, &s, &c);
+    const double2 y0 = cmk(fma(c, x0.x, s * x1.y), fma(c, x0.y, -s * x1.x));
+    const double2 y1 = cmk(fma(c, x1.x, s * x0.y), fma(c, x1.y, -s * x0.x));
     x0 = y0;
     x1 = y1;
 }
 
 // ------------------------------------------------------------------ L-phase
 //
-// One wave covers 64 consecutive positions of one simulation (wave table
-// entry: sim index, first position). Each lane streams over l carrying a
-// three-stage pipeline: virtual row v = l + q (q = parity of j*L) puts the
-// E-pairs on even v and the O-pairs on odd v for every lane, so iteration it
-// loads rows (2it, 2it+1) and finishes rows (2it-3, 2it-2).
+// A wave covers 64 consecutive positions of one simulation and one segment
+// [lo, hi) of its l range (wave table entry: sim, first position, lo, hi).
+// Each lane streams over l carrying a three-stage pipeline: virtual row
+// v = l + q (q = parity of j*L) puts the E-pairs on even v and the O-pairs on
+// odd v for every lane, so iteration it consumes rows (2it, 2it+1) and
+// finishes rows (2it-3, 2it-2). A finished row depends on input rows within
+// distance 3 (O, E, O pair chains), so a segment starts its pipeline 4 rows
+// below lo and stores only rows in [lo, hi); segments let short and long l
+// ranges spread evenly over the GPU. Row loads are issued PF iterations ahead.
 //
-// Mode per simulation, from its step counter n (N = total steps):
-//   n == 0      : start only     E_0, O_0
-//   0 < n < N   : end + start    O_{n-1}, E_{n-1}, mask, E_n, O_n
-//   n == N      : end only       O_{N-1}, E_{N-1}, mask
-// At data times the wavefunction between "end" and "start" (the reference's
-// g after n steps) is observed: per-wave |g|^2 partials and the rows l < LB.
-// end_only != 0 materialises the reference g after n steps (O_{n-1}, E_{n-1},
-// mask) for the listed waves without starting step n and without observing.
-__global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, const int2* __restrict__ waves,
+// Mode per simulation, from its step counter n (N = total steps) and phase:
+//   start      E_n, O_n                          (n < N)
+//   end        O_{n-1}, E_{n-1}, mask            (n > 0 and g is mid-step)
+// At data times the reference g after n steps (between "end" and "start")
+// is observed: per-wave |g|^2 partials and the rows l < LB. Elsewhere the two
+// E rotations of a pair, which share the generator sigma_x and commute with
+// the mask (a function of r only), are applied as one rotation by the sum of
+// their angles. end_only != 0 materialises the reference g after n steps for
+// the listed waves without starting step n and without observing.
+struct ShWave {
+    int sim, p0, lo, hi, part;  // part: index of this wave's norm partial within the simulation
+};
+
+__global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, const ShWave* __restrict__ waves,
                                                    int n_waves, const double* __restrict__ cl, int end_only) {
+    constexpr int PF = 4;  // prefetch depth in iterations (2 rows each)
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / TDSE_WAVE;
     const int lane = threadIdx.x % TDSE_WAVE;
     if (wave >= n_waves) return;
-    const int2 wt = waves[wave];
-    ShSim& S = sims[wt.x];
+    const ShWave wt = waves[wave];
+    ShSim& S = sims[wt.sim];
     const int n = S.step;
     const int N = S.N;
     if (!S.active || n > N) return;
@@ -99,77 +109,117 @@ __global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, con
     const bool do_start = n < N && !end_only;
     const bool observe = !end_only && sh_is_data_step(n, N, S.store_every);
     if (!do_end && !do_start && !observe) return;
+    const bool fuse_e = do_end && do_start && !observe;
 
     const int L = S.L, P = S.P, m = S.m, npad = S.npad;
-    const int p = wt.y + lane;
+    const int lo = wt.lo, hi = wt.hi;
+    const int p = wt.p0 + lane;
     const int j = (p % P) * m + p / P;
     const bool live = j < S.n;
     const int q = (j & 1) & (L & 1);  // parity of j*L
-    const double rq = S.tab[SH_RQ * npad + p];
-    const double mk = S.tab[SH_MASK * npad + p];
-    const double f_end = do_end ? S.fs[n - 1] * rq : 0.0;
-    const double f_start = do_start ? S.fs[n] * rq : 0.0;
+    const double rq = ld1(S.tab, SH_RQ * npad + p);
+    const double mk = ld1(S.tab, SH_MASK * npad + p);
+    const double f_end = do_end ? ld1(S.fs, n - 1) * rq : 0.0;
+    const double f_start = do_start ? ld1(S.fs, n) * rq : 0.0;
+    const double f_sum = f_end + f_start;
     double2* __restrict__ g = S.g + p;
-    const int vmax = L - 1 + q;
+    auto load_row = [g, live, L, npad](int l) -> double2 {
+        double2 v = make_double2(0.0, 0.0);
+        if (live && l >= 0 && l < L) v = ld2(g, (size_t)l * npad);
+        return v;
+    };
+    // coupling of the pair starting at virtual row v (0 when the pair does not exist:
+    // a rotation by 0 is exactly the identity)
+    auto load_c = [cl, q, L](int v) -> double {
+        const int l = v - q;
+        return (l >= 0 && l + 1 < L) ? ld1(cl, l) : 0.0;
+    };
 
-    double2 zero = cmk(0.0, 0.0);
+    // first iteration: rows below lo - 4 do not influence rows >= lo
+    const int it0 = (lo <= 4) ? 0 : ((lo + q - 4) >> 1);
+    const int it_end = (hi - 1 + q + 3) >> 1;  // last iteration stores row hi-1
+    const double2 zero = make_double2(0.0, 0.0);
     double2 carry_u = zero, carry_s1 = zero, carry_s3 = zero;
+    double c_m3 = load_c(2 * it0 - 3), c_m2 = load_c(2 * it0 - 2), c_m1 = load_c(2 * it0 - 1);
     double acc_norm = 0.0, acc_lnorm = 0.0;
+    double2 ring0[PF], ring1[PF];
+    double ringc0[PF], ringc1[PF];
+#pragma unroll
+    for (int d = 0; d < PF; ++d) {
+        ring0[d] = load_row(2 * (it0 + d) - q);
+        ring1[d] = load_row(2 * (it0 + d) + 1 - q);
+        ringc0[d] = load_c(2 * (it0 + d));
+        ringc1[d] = load_c(2 * (it0 + d) + 1);
+    }
 
-    for (int it = 0; 2 * it - 3 <= vmax; ++it) {
+    for (int it = it0; it <= it_end; ++it) {
         const int v0 = 2 * it;
-        const int l0 = v0 - q, l1 = v0 + 1 - q;
-        double2 u0 = (live && l0 >= 0 && l0 < L) ? g[(size_t)l0 * npad] : zero;
-        double2 u1 = (live && l1 >= 0 && l1 < L) ? g[(size_t)l1 * npad] : zero;
-
-        // stage 1: O_end on (v0-1, v0), real pair start l = v0-1-q
-        double2 s1m = carry_u, s10 = u0;
-        {
-            const int lp = v0 - 1 - q;
-            if (do_end && lp >= 0 && lp + 1 < L) sh_rotate(f_end * cl[lp], s1m, s10);
+        const double2 u0 = ring0[0], u1 = ring1[0];
+        const double c0 = ringc0[0], c1 = ringc1[0];
+#pragma unroll
+        for (int d = 0; d + 1 < PF; ++d) {
+            ring0[d] = ring0[d + 1];
+            ring1[d] = ring1[d + 1];
+            ringc0[d] = ringc0[d + 1];
+            ringc1[d] = ringc1[d + 1];
         }
-        // stage 2: E_end on (v0-2, v0-1), then mask
+        ring0[PF - 1] = load_row(v0 + 2 * PF - q);
+        ring1[PF - 1] = load_row(v0 + 2 * PF + 1 - q);
+        ringc0[PF - 1] = load_c(v0 + 2 * PF);
+        ringc1[PF - 1] = load_c(v0 + 2 * PF + 1);
+
+        // stage 1: O_end on (v0-1, v0)
+        double2 s1m = carry_u, s10 = u0;
+        if (do_end) sh_rotate(f_end * c_m1, s1m, s10);
+        // stage 2: E_end on (v0-2, v0-1), mask; stage 3: E_start on the same pair
         double2 s2a = carry_s1, s2b = s1m;
-        const int le = v0 - 2 - q;
-        if (do_end) {
-            if (le >= 0 && le + 1 < L) sh_rotate(f_end * cl[le], s2a, s2b);
+        if (fuse_e) {
+            sh_rotate(f_sum * c_m2, s2a, s2b);
             s2a = cscale(mk, s2a);
             s2b = cscale(mk, s2b);
-        }
-        if (observe && live) {
-            if (le >= 0 && le < L) {
-                acc_norm += cabs2(s2a);
-                if (le == L - 1) acc_lnorm += cabs2(s2a);
-                if (le < S.LB) S.snap[(size_t)le * npad + p] = s2a;
+        } else {
+            if (do_end) {
+                sh_rotate(f_end * c_m2, s2a, s2b);
+                s2a = cscale(mk, s2a);
+                s2b = cscale(mk, s2b);
             }
-            if (le + 1 >= 0 && le + 1 < L) {
-                acc_norm += cabs2(s2b);
-                if (le + 1 == L - 1) acc_lnorm += cabs2(s2b);
-                if (le + 1 < S.LB) S.snap[(size_t)(le + 1) * npad + p] = s2b;
+            const int le = v0 - 2 - q;
+            if (observe && live) {
+                if (le >= lo && le < hi) {
+                    acc_norm += cabs2(s2a);
+                    if (le == L - 1) acc_lnorm += cabs2(s2a);
+                    if (le < S.LB) st2(S.snap, (size_t)le * npad + p, s2a);
+                }
+                if (le + 1 >= lo && le + 1 < hi) {
+                    acc_norm += cabs2(s2b);
+                    if (le + 1 == L - 1) acc_lnorm += cabs2(s2b);
+                    if (le + 1 < S.LB) st2(S.snap, (size_t)(le + 1) * npad + p, s2b);
+                }
             }
+            if (do_start) sh_rotate(f_start * c_m2, s2a, s2b);
         }
-        // stage 3: E_start on the same pair
-        if (do_start && le >= 0 && le + 1 < L) sh_rotate(f_start * cl[le], s2a, s2b);
         // stage 4: O_start on (v0-3, v0-2)
         double2 o_a = carry_s3, o_b = s2a;
-        const int lo = v0 - 3 - q;
-        if (do_start && lo >= 0 && lo + 1 < L) sh_rotate(f_start * cl[lo], o_a, o_b);
+        if (do_start) sh_rotate(f_start * c_m3, o_a, o_b);
+        const int lo_row = v0 - 3 - q;
         if (live) {
-            if (lo >= 0 && lo < L) g[(size_t)lo * npad] = o_a;
-            if (lo + 1 >= 0 && lo + 1 < L) g[(size_t)(lo + 1) * npad] = o_b;
+            if (lo_row >= lo && lo_row < hi) st2(g, (size_t)lo_row * npad, o_a);
+            if (lo_row + 1 >= lo && lo_row + 1 < hi) st2(g, (size_t)(lo_row + 1) * npad, o_b);
         }
         carry_u = u1;
         carry_s1 = s10;
         carry_s3 = s2b;
+        c_m3 = c_m1;
+        c_m2 = c0;
+        c_m1 = c1;
     }
 
     if (observe) {
         acc_norm = wave_sum(acc_norm);
         acc_lnorm = wave_sum(acc_lnorm);
         if (lane == 0) {
-            const int w_in_sim = wt.y / TDSE_WAVE;
-            S.part[w_in_sim] = acc_norm;
-            S.part[S.nwaves + w_in_sim] = acc_lnorm;
+            S.part[wt.part] = acc_norm;
+            S.part[S.nwaves + wt.part] = acc_lnorm;
         }
     }
 }
@@ -195,15 +245,15 @@ __global__ void __launch_bounds__(256) k_sh_observe(ShSim* __restrict__ sims, co
     double re = 0.0, im = 0.0;
     if (jb.y < 0) {
         for (int w = t; w < S.nwaves; w += blockDim.x) {
-            re += S.part[w];
-            im += S.part[S.nwaves + w];
+            re += ld1(S.part, w);
+            im += ld1(S.part, S.nwaves + w);
         }
     } else {
         const double* phi = S.basis + (size_t)jb.y * S.npad;
         const double2* row = S.snap + (size_t)S.basis_l[jb.y] * S.npad;
         for (int p = t; p < S.npad; p += blockDim.x) {
-            const double f = phi[p];
-            const double2 z = row[p];
+            const double f = ld1(phi, p);
+            const double2 z = ld2(row, p);
             re = fma(f, z.x, re);
             im = fma(f, z.y, im);
         }
@@ -233,10 +283,11 @@ __global__ void __launch_bounds__(256) k_sh_observe(ShSim* __restrict__ sims, co
 // (1 + i H~) y = (1 - i H~) x along r for one l-line, H~ = tau_n H0 real
 // symmetric tridiagonal: diagonal s_n (T_j + l(l+1) U_j [+ corr0 at l = 0,
 // j = 0]), off-diagonals s_n A_{j-1}, s_n A_j (mesh_operators.py:889-928,
-// 244-269). One workgroup of P threads per line; the solve is cn_partition.
+// 244-269). One workgroup of P threads per line, M (= m) rows per thread;
+// the solve is cn_cayley_solve: z = (1 + i H~)^-1 x, y = 2 z - x.
 template <int M, int MAXT>
 __global__ void __launch_bounds__(MAXT) k_sh_rphase(ShSim* __restrict__ sims, const int2* __restrict__ lines,
-                                                   int n_lines) {
+                                                    int n_lines) {
     extern __shared__ double2 lds[];  // 3 * P
     const int line = blockIdx.x;
     if (line >= n_lines) return;
@@ -245,9 +296,9 @@ __global__ void __launch_bounds__(MAXT) k_sh_rphase(ShSim* __restrict__ sims, co
     const int n = S.step;
     if (!S.active || n >= S.N) return;  // uniform across the block
     const int l = lt.y;
-    const int P = S.P, m = S.m, npad = S.npad;
+    const int P = S.P, npad = S.npad;
     const int k = threadIdx.x;
-    const double sn = S.ts[n];
+    const double sn = ld1(S.ts, n);
     const double ll1 = (double)l * (double)(l + 1);
     const double c0 = (l == 0 && k == 0) ? S.corr0 : 0.0;
     double2* __restrict__ gl = S.g + (size_t)l * npad + k;
@@ -258,20 +309,18 @@ __global__ void __launch_bounds__(MAXT) k_sh_rphase(ShSim* __restrict__ sims, co
 
     double2 x[M];
 #pragma unroll
-    for (int i = 0; i < M; ++i)
-        if (i < m) x[i] = gl[(size_t)i * P];
-    const double2 xlast = gl[(size_t)(m - 1) * P];
-    double2 xl, xr;
-    chunk_neighbours<M>(x, xlast, k, P, lds, xl, xr, false);
+    for (int i = 0; i < M; ++i) x[i] = ld2(gl, (size_t)i * P);
     auto coef = [&](int i, double& hm, double& h, double& hp) {
-        h = sn * (fma(ll1, U[i * P], T[i * P]) + (i == 0 ? c0 : 0.0));
-        hm = sn * AP[i * P];
-        hp = sn * AN[i * P];
+        h = sn * (fma(ll1, ld1(U, i * P), ld1(T, i * P)) + (i == 0 ? c0 : 0.0));
+        hm = sn * ld1(AP, i * P);
+        hp = sn * ld1(AN, i * P);
     };
-    cn_partition<M, true>(x, m, k, P, xl, xr, cmk(0.0, 1.0), coef, lds, true);
+    cn_cayley_solve<M>(x, k, P, coef, lds);
 #pragma unroll
-    for (int i = 0; i < M; ++i)
-        if (i < m) gl[(size_t)i * P] = x[i];
+    for (int i = 0; i < M; ++i) {
+        const double2 x0 = ld2(gl, (size_t)i * P);
+        st2(gl, (size_t)i * P, cmk(fma(2.0, x[i].x, -x0.x), fma(2.0, x[i].y, -x0.y)));
+    }
 }
 
 // Advance every active simulation's step counter after a full graph step.

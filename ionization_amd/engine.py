@@ -39,10 +39,10 @@ def choose_layout(n: int, kind: str = "sh", rows: Optional[int] = None) -> Tuple
     """
     Pick (P, m) for a line of n points: P a multiple of 64, m even, P m >= n,
     within the kernel limits (SH: m <= 8 -> P <= 512, m <= 16 -> P <= 256;
-    Line: m <= 8 -> P <= 1024, m <= 16 -> P <= 512). Prefers m = 8, the
+    Line: P <= 512). Prefers m = 8, the
     variant with the lighter register footprint, and the least padding.
     """
-    limits = {"sh": {8: 512, 16: 256}, "line": {8: 1024, 16: 512}}[kind]
+    limits = {"sh": {8: 512, 16: 256}, "line": {8: 512, 16: 512}}[kind]
     cands = [rows] if rows is not None else [8, 6, 4, 16, 14, 12, 10, 2]
     best = None
     for m in cands:
