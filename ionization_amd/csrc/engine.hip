@@ -270,23 +270,23 @@ void prof_end(tdse_ctx* ctx) {
     (void)hipEventRecord(next_event(ctx), ctx->stream);
 }
 
-template <int M, int MAXT>
+template <int M, int MAXT, int WPE>
 void launch_rphase(tdse_ctx* ctx, const Bucket& b) {
     const size_t lds = 3 * (size_t)b.P * sizeof(double2);
-    hipLaunchKernelGGL((k_sh_rphase<M, MAXT>), dim3((unsigned)b.lines.count), dim3(b.P), lds, ctx->stream, ctx->d_sh,
+    hipLaunchKernelGGL((k_sh_rphase<M, MAXT, WPE>), dim3((unsigned)b.lines.count), dim3(b.P), lds, ctx->stream, ctx->d_sh,
                        b.lines.ptr, (int)b.lines.count);
 }
 
 void dispatch_rphase(tdse_ctx* ctx, const Bucket& b) {
     switch (b.M) {
-        case 2: launch_rphase<2, 512>(ctx, b); break;
-        case 4: launch_rphase<4, 512>(ctx, b); break;
-        case 6: launch_rphase<6, 512>(ctx, b); break;
-        case 8: launch_rphase<8, 512>(ctx, b); break;
-        case 10: launch_rphase<10, 256>(ctx, b); break;
-        case 12: launch_rphase<12, 256>(ctx, b); break;
-        case 14: launch_rphase<14, 256>(ctx, b); break;
-        default: launch_rphase<16, 256>(ctx, b); break;
+        case 2: launch_rphase<2, 512, 4>(ctx, b); break;
+        case 4: launch_rphase<4, 512, 4>(ctx, b); break;
+        case 6: launch_rphase<6, 512, 4>(ctx, b); break;
+        case 8: launch_rphase<8, 512, 4>(ctx, b); break;
+        case 10: launch_rphase<10, 256, 3>(ctx, b); break;
+        case 12: launch_rphase<12, 256, 2>(ctx, b); break;
+        case 14: launch_rphase<14, 256, 2>(ctx, b); break;
+        default: launch_rphase<16, 256, 1>(ctx, b); break;
     }
 }
 

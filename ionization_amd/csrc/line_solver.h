@@ -14,8 +14,9 @@
 //  * A running backward pass expresses row 0 through (L_k, Y_k),
 //    Y_k = y_{k m + m - 1}; the chunk's last row then gives one equation in
 //    (Y_{k-1}, Y_k, Y_{k+1}).
-//  * The P-unknown reduced system is solved by parallel cyclic reduction in
-//    LDS with every equation normalised to a unit diagonal.
+//  * The P-unknown reduced system (every equation normalised to a unit
+//    diagonal) is solved in LDS by cyclic reduction (reduced_cr) or parallel
+//    cyclic reduction (reduced_pcr).
 //  * Each thread back-substitutes its rows.
 //
 // M is the exact number of rows per chunk (a compile-time constant, 2 <= M).
@@ -34,6 +35,102 @@ __device__ __forceinline__ double2 wmul(double2 w, double2 v) {
 
 // coef(i, hm, h, hp): H_{j,j-1}, H_{j,j}, H_{j,j+1} of the thread's row i.
 // xl / xr: x_{km-1} and x_{km+m} (zero at the line ends).
+// The reduced system  a_k Y_{k-1} + Y_k + c_k Y_{k+1} = d_k,  k < P  (one equation
+// per thread, unit diagonal, a_0 = c_{P-1} = 0). Both solvers return Y_k and Y_{k-1}.
+//
+// Parallel cyclic reduction: ceil(log2 P) levels, every equation updated at every level.
+__device__ __forceinline__ void reduced_pcr(double2 A, double2 C, double2 D, const int k, const int P,
+                                            double2* __restrict__ lds, double2& Yk, double2& Ykm1) {
+    double2* pa = lds;
+    double2* pc = lds + P;
+    double2* pd = lds + 2 * P;
+    for (int s = 1; s < P; s <<= 1) {
+        __syncthreads();
+        pa[k] = A;
+        pc[k] = C;
+        pd[k] = D;
+        __syncthreads();
+        double2 am = cmk(0.0, 0.0), cm = cmk(0.0, 0.0), dm = cmk(0.0, 0.0);
+        double2 apl = cmk(0.0, 0.0), cpl = cmk(0.0, 0.0), dpl = cmk(0.0, 0.0);
+        if (k - s >= 0) {
+            am = pa[k - s];
+            cm = pc[k - s];
+            dm = pd[k - s];
+        }
+        if (k + s < P) {
+            apl = pa[k + s];
+            cpl = pc[k + s];
+            dpl = pd[k + s];
+        }
+        const double2 bn = csub(csub(cmk(1.0, 0.0), cmul(A, cm)), cmul(C, apl));
+        const double2 an2 = cneg(cmul(A, am));
+        const double2 cn2 = cneg(cmul(C, cpl));
+        const double2 dn2 = csub(csub(D, cmul(A, dm)), cmul(C, dpl));
+        const double2 ib = crecip(bn);
+        A = cmul(an2, ib);
+        C = cmul(cn2, ib);
+        D = cmul(dn2, ib);
+    }
+    __syncthreads();
+    pd[k] = D;
+    __syncthreads();
+    Yk = D;
+    Ykm1 = (k > 0) ? pd[k - 1] : cmk(0.0, 0.0);
+}
+
+// Cyclic reduction with compacted threads: at stride s the surviving equations
+// kk = 2 s (t + 1) - 1 are updated by threads t = 0, 1, ... (so whole waves drop
+// out as the system shrinks), eliminating kk - s and kk + s; the back
+// substitution then solves the eliminated equations s (2 t + 1) - 1 from their
+// already solved neighbours. About 2 P equation updates in total instead of
+// P log2 P, at the same number of barriers.
+__device__ __forceinline__ void reduced_cr(double2 A, double2 C, double2 D, const int k, const int P,
+                                           double2* __restrict__ lds, double2& Yk, double2& Ykm1) {
+    double2* pa = lds;
+    double2* pc = lds + P;
+    double2* pd = lds + 2 * P;
+    __syncthreads();
+    pa[k] = A;
+    pc[k] = C;
+    pd[k] = D;
+    int s = 1;
+    for (; s < P; s <<= 1) {
+        __syncthreads();
+        const int kk = 2 * s * (k + 1) - 1;
+        if (kk < P) {
+            const double2 a0 = pa[kk], c0 = pc[kk], d0 = pd[kk];
+            const double2 am = pa[kk - s], cm = pc[kk - s], dm = pd[kk - s];
+            double2 apl = cmk(0.0, 0.0), cpl = cmk(0.0, 0.0), dpl = cmk(0.0, 0.0);
+            if (kk + s < P) {
+                apl = pa[kk + s];
+                cpl = pc[kk + s];
+                dpl = pd[kk + s];
+            }
+            const double2 bn = csub(csub(cmk(1.0, 0.0), cmul(a0, cm)), cmul(c0, apl));
+            const double2 an2 = cneg(cmul(a0, am));
+            const double2 cn2 = cneg(cmul(c0, cpl));
+            const double2 dn2 = csub(csub(d0, cmul(a0, dm)), cmul(c0, dpl));
+            const double2 ib = crecip(bn);
+            pa[kk] = cmul(an2, ib);
+            pc[kk] = cmul(cn2, ib);
+            pd[kk] = cmul(dn2, ib);
+        }
+    }
+    // equations still standing have no neighbours in range: their d is their solution
+    for (s >>= 1; s >= 1; s >>= 1) {
+        __syncthreads();
+        const int kk = s * (2 * k + 1) - 1;
+        if (kk < P) {
+            const double2 ym = (kk - s >= 0) ? pd[kk - s] : cmk(0.0, 0.0);
+            const double2 yp = (kk + s < P) ? pd[kk + s] : cmk(0.0, 0.0);
+            pd[kk] = csub(csub(pd[kk], cmul(pa[kk], ym)), cmul(pc[kk], yp));
+        }
+    }
+    __syncthreads();
+    Yk = pd[k];
+    Ykm1 = (k > 0) ? pd[k - 1] : cmk(0.0, 0.0);
+}
+
 // Off-diagonal coefficients w*hm, w*hp: complex in general; for IMAG (w = i s)
 // they are i times a real number and are kept as that real number.
 template <bool IMAG>
@@ -59,7 +156,7 @@ struct OffDiag<true> {
 
 // RHS_IS_X: the right-hand side is x itself (implicit solve only, no explicit
 // half); xl / xr are then unused.
-template <int M, bool IMAG, class Coef, bool RHS_IS_X = false>
+template <int M, bool IMAG, class Coef, bool RHS_IS_X = false, bool REDUCED_CR = true>
 __device__ __forceinline__ void cn_partition(double2 (&x)[M], const int k, const int P,
                                              const double2 xl, const double2 xr, const double2 w,
                                              Coef coef, double2* __restrict__ lds, const bool sync_first) {
@@ -139,40 +236,10 @@ __device__ __forceinline__ void cn_partition(double2 (&x)[M], const int k, const
         C = cmul(C, ib);
         D = cmul(D, ib);
     }
-    double2* pa = lds;
-    double2* pc = lds + P;
-    double2* pd = lds + 2 * P;
-    for (int s = 1; s < P; s <<= 1) {
-        __syncthreads();
-        pa[k] = A;
-        pc[k] = C;
-        pd[k] = D;
-        __syncthreads();
-        double2 am = cmk(0.0, 0.0), cm = cmk(0.0, 0.0), dm = cmk(0.0, 0.0);
-        double2 apl = cmk(0.0, 0.0), cpl = cmk(0.0, 0.0), dpl = cmk(0.0, 0.0);
-        if (k - s >= 0) {
-            am = pa[k - s];
-            cm = pc[k - s];
-            dm = pd[k - s];
-        }
-        if (k + s < P) {
-            apl = pa[k + s];
-            cpl = pc[k + s];
-            dpl = pd[k + s];
-        }
-        const double2 bn = csub(csub(cmk(1.0, 0.0), cmul(A, cm)), cmul(C, apl));
-        const double2 an2 = cneg(cmul(A, am));
-        const double2 cn2 = cneg(cmul(C, cpl));
-        const double2 dn2 = csub(csub(D, cmul(A, dm)), cmul(C, dpl));
-        const double2 ib = crecip(bn);
-        A = cmul(an2, ib);
-        C = cmul(cn2, ib);
-        D = cmul(dn2, ib);
-    }
-    __syncthreads();
-    pd[k] = D;  // Y_k
-    __syncthreads();
-    const double2 Lk = (k > 0) ? pd[k - 1] : cmk(0.0, 0.0);
+    double2 Lk, Yk;
+    if constexpr (REDUCED_CR) reduced_cr(A, C, D, k, P, lds, Yk, Lk);
+    else reduced_pcr(A, C, D, k, P, lds, Yk, Lk);
+    D = Yk;
     double2 y = D;
 #pragma unroll
     for (int i = M - 1; i >= 0; --i) {
@@ -191,6 +258,102 @@ template <int M, class Coef>
 __device__ __forceinline__ void cn_cayley_solve(double2 (&x)[M], const int k, const int P, Coef coef,
                                                 double2* __restrict__ lds) {
     cn_partition<M, true, Coef, true>(x, k, P, cmk(0.0, 0.0), cmk(0.0, 0.0), cmk(0.0, 1.0), coef, lds, false);
+}
+
+// Implicit solve (1 + i H) z = x for real H with a smaller register footprint
+// than cn_partition: per row only gamma and delta are kept (4 doubles).
+//
+//  pass 1 (forward): pivots r_i, gamma_i = r_i sup_i, delta_i = r_i (x_i - sub_i delta_{i-1})
+//          and the running alpha_i = -r_i sub_i alpha_{i-1}; the row-0 coefficients
+//          through (L_k, Y_k) that the previous chunk needs are accumulated forward,
+//          alpha'_0 = sum_i pi_i alpha_i, delta'_0 = sum_i pi_i delta_i,
+//          gamma'_0 = pi_{m-2} gamma_{m-2}, with pi_i = prod_{k<i} (-gamma_k);
+//  reduced system (one equation per chunk, unit diagonal) by cyclic reduction;
+//  pass 2 (forward): the left-boundary term mu_i = alpha_i L_k = -r_i sub_i mu_{i-1},
+//          pivots recomputed from the stored gamma, delta_i <- delta_i - mu_i;
+//  pass 3 (backward): z_{m-1} = Y_k, z_i = delta_i - gamma_i z_{i+1}.
+// coef(i, hm, h, hp) as for cn_partition; sub = i hm, diag = 1 + i h, sup = i hp.
+// coef2 is called in pass 2 and must re-read the coefficients (not reuse
+// pass-1 values held in registers across the reduced solve).
+template <int M, class Coef, class Coef2>
+__device__ __forceinline__ void cn_implicit_lowreg(double2 (&x)[M], const int k, const int P, Coef coef,
+                                                   Coef2 coef2, double2* __restrict__ lds) {
+    double2 ga[M - 1], de[M - 1];
+    double2 al = cmk(0.0, 0.0), pi = cmk(1.0, 0.0), sa = cmk(0.0, 0.0), sd = cmk(0.0, 0.0);
+    double2 gprev = cmk(0.0, 0.0), dprev = cmk(0.0, 0.0);
+    double2 dlast = cmk(0.0, 0.0), diaglast = cmk(1.0, 0.0);
+    double sublast = 0.0, suplast = 0.0;
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        double hm, h, hp;
+        coef(i, hm, h, hp);
+        if (i < M - 1) {
+            // den = (1 + i h) - (i hm) gamma_{i-1}
+            const double2 den = (i == 0) ? cmk(1.0, h) : cmk(fma(hm, gprev.y, 1.0), fma(-hm, gprev.x, h));
+            const double2 r = crecip(den);
+            const double2 rsub = cimul(hm, r);
+            al = (i == 0) ? rsub : cneg(cmul(rsub, al));
+            const double2 d = (i == 0) ? cmul(r, x[0]) : csub(cmul(r, x[i]), cmul(rsub, dprev));
+            const double2 g = cimul(hp, r);
+            sa = cadd(sa, cmul(pi, al));
+            sd = cadd(sd, cmul(pi, d));
+            if (i < M - 2) pi = cneg(cmul(pi, g));
+            ga[i] = g;
+            de[i] = d;
+            gprev = g;
+            dprev = d;
+        } else {
+            dlast = x[i];
+            diaglast = cmk(1.0, h);
+            sublast = hm;
+            suplast = hp;
+        }
+    }
+    const double2 gp0 = cmul(pi, gprev);  // pi_{m-2} gamma_{m-2}
+    __syncthreads();
+    lds[k] = sa;
+    lds[P + k] = gp0;
+    lds[2 * P + k] = sd;
+    __syncthreads();
+    double2 na = cmk(0.0, 0.0), ng = cmk(0.0, 0.0), nd = cmk(0.0, 0.0);
+    if (k + 1 < P) {
+        na = lds[k + 1];
+        ng = lds[P + k + 1];
+        nd = lds[2 * P + k + 1];
+    }
+    // reduced equation of the chunk's last row: A Y_{k-1} + B Y_k + C Y_{k+1} = D
+    double2 A = cneg(cimul(sublast, al));
+    const double2 B = csub(csub(diaglast, cimul(sublast, gprev)), cimul(suplast, na));
+    double2 C = cneg(cimul(suplast, ng));
+    double2 D = csub(csub(dlast, cimul(sublast, dprev)), cimul(suplast, nd));
+    {
+        const double2 ib = crecip(B);
+        A = cmul(A, ib);
+        C = cmul(C, ib);
+        D = cmul(D, ib);
+    }
+    double2 Lk, Yk;
+    reduced_cr(A, C, D, k, P, lds, Yk, Lk);
+    // pass 2: subtract alpha_i L_k
+    double2 mu = cmk(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < M - 1; ++i) {
+        double hm, h;
+        coef2(i, hm, h);
+        const double2 den = (i == 0) ? cmk(1.0, h) : cmk(fma(hm, ga[i - 1 > 0 ? i - 1 : 0].y, 1.0),
+                                                          fma(-hm, ga[i - 1 > 0 ? i - 1 : 0].x, h));
+        const double2 rsub = cimul(hm, crecip(den));
+        mu = (i == 0) ? cmul(rsub, Lk) : cneg(cmul(rsub, mu));
+        de[i] = csub(de[i], mu);
+    }
+    // pass 3: back substitution
+    double2 z = Yk;
+    x[M - 1] = z;
+#pragma unroll
+    for (int i = M - 2; i >= 0; --i) {
+        z = csub(de[i], cmul(ga[i], z));
+        x[i] = z;
+    }
 }
 
 // Exchange the first / last rows of every chunk through LDS so each thread

@@ -284,10 +284,13 @@ __global__ void __launch_bounds__(256) k_sh_observe(ShSim* __restrict__ sims, co
 // symmetric tridiagonal: diagonal s_n (T_j + l(l+1) U_j [+ corr0 at l = 0,
 // j = 0]), off-diagonals s_n A_{j-1}, s_n A_j (mesh_operators.py:889-928,
 // 244-269). One workgroup of P threads per line, M (= m) rows per thread;
-// the solve is cn_cayley_solve: z = (1 + i H~)^-1 x, y = 2 z - x.
-template <int M, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_sh_rphase(ShSim* __restrict__ sims, const int2* __restrict__ lines,
-                                                    int n_lines) {
+// the solve is cn_implicit_lowreg: z = (1 + i H~)^-1 x, y = 2 z - x. The
+// solver keeps two complex values per row between its passes and re-reads the
+// (L2-resident) coefficient tables instead of holding them, so WPE waves fit
+// per SIMD; that occupancy hides the HBM latency of the line loads.
+template <int M, int MAXT, int WPE>
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+k_sh_rphase(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines) {
     extern __shared__ double2 lds[];  // 3 * P
     const int line = blockIdx.x;
     if (line >= n_lines) return;
@@ -315,7 +318,18 @@ __global__ void __launch_bounds__(MAXT) k_sh_rphase(ShSim* __restrict__ sims, co
         hm = sn * ld1(AP, i * P);
         hp = sn * ld1(AN, i * P);
     };
-    cn_cayley_solve<M>(x, k, P, coef, lds);
+    // Second read of the same coefficients. The empty asm makes the pointers
+    // opaque so the compiler reloads instead of keeping the first pass's
+    // values live across the solver's barriers.
+    auto coef2 = [&](int i, double& hm, double& h) {
+        const double* T2 = T;
+        const double* U2 = U;
+        const double* AP2 = AP;
+        asm volatile("" : "+v"(T2), "+v"(U2), "+v"(AP2));
+        h = sn * (fma(ll1, ld1(U2, i * P), ld1(T2, i * P)) + (i == 0 ? c0 : 0.0));
+        hm = sn * ld1(AP2, i * P);
+    };
+    cn_implicit_lowreg<M>(x, k, P, coef, coef2, lds);
 #pragma unroll
     for (int i = 0; i < M; ++i) {
         const double2 x0 = ld2(gl, (size_t)i * P);
