@@ -6,6 +6,8 @@ step) versus the plain path (one simulation, direct kernel launches, no
 graph), complex128, on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --calibrate-copy ...      (profiling runs: adds a copy of known size)
+    python bench.py --summarize-profiles DIR  (CPU: rocprofv3 databases -> profiles/)
 
 Prints one JSON line. ``value`` is the batched throughput with every
 wavefunction resident in HBM; ``roofline`` is for the kernel with the largest
@@ -27,6 +29,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+COPY_CAL_BYTES = 1 << 30  # read and written once per calibration copy launch
+COPY_CAL_LAUNCHES = 3
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sh_scan_hbm_traffic.txt")
 BYTES_PER_POINT_STEP = 64  # two complex128 read+write sweeps (L-phase, R-phase)
 BYTES_PER_POINT_PHASE = 32
 
@@ -113,6 +118,127 @@ def cpu_baseline(params, seconds):
                 sample=f"scan simulation 0 ({p['l_bound']} l x {p['r_points']} r), {steps} steps")
 
 
+def calibrate_copy():
+    """The library's streaming copy of COPY_CAL_BYTES (k_stream_copy): the
+    known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters to bytes;
+    also the attainable copy bandwidth."""
+    from ionization_amd.engine import DeviceContext
+
+    ctx = DeviceContext(pool_bytes=2 * COPY_CAL_BYTES + (16 << 20))
+    ms = ctx.copy_probe(COPY_CAL_BYTES, COPY_CAL_LAUNCHES)
+    ctx.close()
+    gbs = 2 * COPY_CAL_BYTES / (ms * 1e-3) / 1e9
+    log(f"copy calibration: {COPY_CAL_LAUNCHES} x ({COPY_CAL_BYTES} B read + {COPY_CAL_BYTES} B written), "
+        f"{ms:.3f} ms each = {gbs:.0f} GB/s")
+    return gbs
+
+
+def load_traffic():
+    """HBM bytes per launch measured by the committed counter summary:
+    lines 'traffic <kernel> <fetch_bytes> <write_bytes>' (per launch, calibrated)."""
+    out = {}
+    if not os.path.exists(TRAFFIC_FILE):
+        return out
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 4 and parts[0] == "traffic":
+                out[parts[1]] = (float(parts[2]), float(parts[3]))
+    return out
+
+
+def _kernel_family(name):
+    base = name.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "").strip()
+    return base.split("<")[0]
+
+
+def summarize_profiles(d, command_trace, command_fetch, command_write):
+    """Reads rocprofv3 databases under d/{trace,fetch,write} and writes the
+    kernel-time and HBM-traffic summaries under profiles/."""
+    import glob
+    import sqlite3
+
+    def db(sub):
+        files = sorted(glob.glob(os.path.join(d, sub, "**", "*_results.db"), recursive=True))
+        if not files:
+            raise SystemExit(f"no rocprofv3 database under {os.path.join(d, sub)}")
+        return sqlite3.connect(files[0])
+
+    prof_dir = os.path.join(ROOT, "profiles")
+    os.makedirs(prof_dir, exist_ok=True)
+
+    con = db("trace")
+    rows = con.execute("select name, count(*), sum(duration), avg(duration), min(duration), max(duration) "
+                       "from kernels group by name order by sum(duration) desc").fetchall()
+    fam = {}
+    for name, cnt, tot, avg, mn, mx in rows:
+        f = _kernel_family(name)
+        c, t = fam.get(f, (0, 0.0))
+        fam[f] = (c + cnt, t + tot)
+    lines = ["# rocprofv3 kernel trace: 256-simulation SphericalHarmonicMesh scan (bench.py)", "",
+             "command:", "    " + command_trace, "",
+             f"{'kernel':<70} {'calls':>7} {'total ms':>10} {'avg us':>9} {'min us':>9} {'max us':>9}"]
+    for name, cnt, tot, avg, mn, mx in rows:
+        if tot < 1e5:
+            continue
+        lines.append(f"{name[:70]:<70} {cnt:>7d} {tot / 1e6:>10.3f} {avg / 1e3:>9.1f} {mn / 1e3:>9.1f} {mx / 1e3:>9.1f}")
+    lines += ["", "per kernel family (all template instances):",
+              f"{'family':<30} {'calls':>7} {'total ms':>10} {'avg us':>9}"]
+    for f, (c, t) in sorted(fam.items(), key=lambda kv: -kv[1][1]):
+        if t < 1e5:
+            continue
+        lines.append(f"{f:<30} {c:>7d} {t / 1e6:>10.3f} {t / c / 1e3:>9.1f}")
+    with open(os.path.join(prof_dir, "sh_scan_kernel_trace.txt"), "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+    def counter(sub, cname):
+        con = db(sub)
+        q = ("select kernel_name, count(*), sum(value) from counters_collection "
+             "where counter_name = ? group by kernel_name")
+        return {k: (c, v * 1024.0) for k, c, v in con.execute(q, (cname,)).fetchall()}  # KiB -> B
+
+    fetch = counter("fetch", "FETCH_SIZE")
+    write = counter("write", "WRITE_SIZE")
+
+    def cal(table):
+        cands = [(k, c, v) for k, (c, v) in table.items() if _kernel_family(k) == "k_stream_copy"]
+        if not cands:
+            raise SystemExit("calibration copy kernel not found (run the profiled command with --calibrate-copy)")
+        k, c, v = cands[0]
+        return COPY_CAL_BYTES * c / v, k
+
+    f_scale, f_kernel = cal(fetch)
+    w_scale, w_kernel = cal(write)
+    fams = {}
+    for table, idx in ((fetch, 0), (write, 1)):
+        for k, (c, v) in table.items():
+            f = _kernel_family(k)
+            if not f.startswith("k_"):
+                continue
+            e = fams.setdefault(f, [0, 0.0, 0, 0.0])
+            e[2 * idx] += c
+            e[2 * idx + 1] += v
+    lines = ["# HBM traffic per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE (KiB per dispatch),",
+             "# scaled by what the same counters read for a float64 copy of 1 GiB (read 1 GiB, write 1 GiB).", "",
+             "commands:", "    " + command_fetch, "    " + command_write, "",
+             f"calibration kernel: {f_kernel[:90]}",
+             f"FETCH_SIZE scale = {f_scale:.4f} (1 GiB / counter reading), WRITE_SIZE scale = {w_scale:.4f}", "",
+             f"{'kernel family':<20} {'launches':>8} {'fetch MB/launch':>16} {'write MB/launch':>16}"]
+    traffic = []
+    for f, (cf, vf, cw, vw) in sorted(fams.items()):
+        if cf == 0 or cw == 0:
+            continue
+        fb = vf / cf * f_scale
+        wb = vw / cw * w_scale
+        lines.append(f"{f:<20} {cf:>8d} {fb / 1e6:>16.2f} {wb / 1e6:>16.2f}")
+        traffic.append(f"traffic {f} {fb:.6e} {wb:.6e}")
+    lines += ["", "machine-readable (bytes per launch, calibrated):"] + traffic
+    with open(TRAFFIC_FILE, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    print(open(os.path.join(prof_dir, "sh_scan_kernel_trace.txt")).read())
+    print(open(TRAFFIC_FILE).read())
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,7 +249,15 @@ def main():
     ap.add_argument("--store-every", type=int, default=100)
     ap.add_argument("--plain-steps", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--summarize-profiles", metavar="DIR")
+    ap.add_argument("--trace-command", default="")
+    ap.add_argument("--fetch-command", default="")
+    ap.add_argument("--write-command", default="")
     args = ap.parse_args()
+    if args.summarize_profiles:
+        summarize_profiles(args.summarize_profiles, args.trace_command, args.fetch_command, args.write_command)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (simulations shard by "
               "simulation with no exchange, see the design notes)", file=sys.stderr)
@@ -136,9 +270,11 @@ def main():
     b = time_batched(compiled, args.steps, args.warmup)
     rate = b["points"] * args.steps / b["wall_s"]
     log(f"batched: {b['wall_s'] * 1e3 / args.steps:.3f} ms/step, {rate / 1e9:.2f} G mesh-point-steps/s")
-    pl = time_plain(args.plain_steps, 10)
-    plain_rate = pl["points"] * args.plain_steps / pl["wall_s"]
-    log(f"plain: {pl['wall_s'] * 1e3 / args.plain_steps:.3f} ms/step, {plain_rate / 1e9:.3f} G mesh-point-steps/s")
+    plain_rate = None
+    if args.plain_steps > 0:  # 0 skips the plain pass (profiling runs)
+        pl = time_plain(args.plain_steps, 10)
+        plain_rate = pl["points"] * args.plain_steps / pl["wall_s"]
+        log(f"plain: {pl['wall_s'] * 1e3 / args.plain_steps:.3f} ms/step, {plain_rate / 1e9:.3f} G mesh-point-steps/s")
 
     prof = b["prof"]
     l_avg = prof["lphase_ms"] / max(1, prof["lphase_launches"])
@@ -153,6 +289,8 @@ def main():
     dom_name = max(kern, key=lambda k: kern[k]["total_ms"])
     dom = kern[dom_name]
     achieved = dom["bytes"] / (dom["avg_ms"] * 1e-3) / 1e9
+    measured = load_traffic().get(dom_name)
+    traffic = None if measured is None else measured[0] + measured[1]
     step_gbs = rate * BYTES_PER_POINT_STEP / 1e9
 
     cpu = None
@@ -183,7 +321,7 @@ def main():
             "plain_workload": "sh_single: 200 l x 1000 r, one simulation, direct launches, no graph",
         },
         "plain_value": plain_rate,
-        "batched_over_plain": rate / plain_rate,
+        "batched_over_plain": None if plain_rate is None else rate / plain_rate,
         "device_ms_per_step": b["dev_ms"] / args.steps,
         "step_hbm_gbs": step_gbs,
         "step_hbm_frac": step_gbs / HBM_PEAK_GBS,
@@ -196,11 +334,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
         },
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.calibrate_copy:  # after the JSON line: a known-size kernel for the counter passes
+        calibrate_copy()
 
 
 if __name__ == "__main__":

@@ -48,6 +48,15 @@ int tdse_sync(tdse_ctx *ctx);
 int tdse_pool_usage(tdse_ctx *ctx, size_t *used_bytes, size_t *total_bytes);
 
 /*
+ * Streaming copy of `bytes` (a multiple of 16) between two blocks carved from
+ * the context pool, `launches` times; *avg_ms = average kernel time from HIP
+ * events on the context stream. No counterpart in the reference: it gives the
+ * attainable copy bandwidth and the known-size kernel that scales HBM byte
+ * counters (FETCH_SIZE / WRITE_SIZE) to bytes.
+ */
+int tdse_copy_probe(tdse_ctx *ctx, size_t bytes, int launches, double *avg_ms);
+
+/*
  * Tridiagonal solve x = T^-1 d for one complex128 system of length n >= 1,
  * sub[i] multiplying x[i-1] (sub[0] ignored) and sup[i] multiplying x[i+1]
  * (sup[n-1] ignored). Host arrays in, host array out; inputs unchanged.

@@ -56,6 +56,7 @@ SIGNATURES = {
     "tdse_ctx_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "tdse_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "tdse_pool_usage": (ctypes.c_int, [ctypes.c_void_p, c_size_t_p, c_size_t_p]),
+    "tdse_copy_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, c_double_p]),
     "tdse_tdma": (ctypes.c_int, [ctypes.c_void_p] + [c_double_p] * 5 + [ctypes.c_int64]),
     "tdse_tdma_batched": (
         ctypes.c_int,

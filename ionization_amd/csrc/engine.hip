@@ -270,10 +270,12 @@ void prof_end(tdse_ctx* ctx) {
     (void)hipEventRecord(next_event(ctx), ctx->stream);
 }
 
+// Launch shapes per rows-per-thread M (block size, minimum waves per EU,
+// explicit or Cayley form), chosen from kernel-trace timings of the scan.
 template <int M, int MAXT, int WPE>
 void launch_rphase(tdse_ctx* ctx, const Bucket& b) {
     const size_t lds = 3 * (size_t)b.P * sizeof(double2);
-    hipLaunchKernelGGL((k_sh_rphase<M, MAXT, WPE>), dim3((unsigned)b.lines.count), dim3(b.P), lds, ctx->stream, ctx->d_sh,
+    hipLaunchKernelGGL((k_sh_rphase<M, MAXT, WPE, (M >= 10)>), dim3((unsigned)b.lines.count), dim3(b.P), lds, ctx->stream, ctx->d_sh,
                        b.lines.ptr, (int)b.lines.count);
 }
 
@@ -367,6 +369,13 @@ int copy_to_device(tdse_ctx* ctx, std::vector<void*>& blocks, const T* host, siz
     return TDSE_OK;
 }
 
+// Plain streaming copy: one complex128 read and one write per element.
+__global__ void __launch_bounds__(256) k_stream_copy(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                     size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) st2(dst, i, ld2(src, i));
+}
+
 void free_blocks(tdse_ctx* ctx, std::vector<void*>& blocks) {
     for (void* p : blocks) ctx->arena.dealloc(p);
     blocks.clear();
@@ -451,6 +460,35 @@ int tdse_pool_usage(tdse_ctx* ctx, size_t* used_bytes, size_t* total_bytes) {
     if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
     if (used_bytes) *used_bytes = ctx->arena.used;
     if (total_bytes) *total_bytes = ctx->arena.size;
+    return TDSE_OK;
+}
+
+int tdse_copy_probe(tdse_ctx* ctx, size_t bytes, int launches, double* avg_ms) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    if (bytes == 0 || bytes % sizeof(double2) != 0 || launches < 1)
+        return fail(ctx, TDSE_ERR_ARG, "tdse_copy_probe: bytes must be a positive multiple of 16, launches >= 1");
+    std::vector<void*> blocks;
+    double2 *src, *dst;
+    const size_t n = bytes / sizeof(double2);
+    int rc = copy_to_device<double2>(ctx, blocks, nullptr, n, &src);
+    if (rc == TDSE_OK) rc = copy_to_device<double2>(ctx, blocks, nullptr, n, &dst);
+    if (rc != TDSE_OK) {
+        free_blocks(ctx, blocks);
+        return rc;
+    }
+    const unsigned grid = 256 * 32;
+    hipError_t e = hipEventRecord(ctx->ev_start, ctx->stream);
+    for (int i = 0; i < launches && e == hipSuccess; ++i) {
+        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, src, dst, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev_stop, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev_stop);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+    free_blocks(ctx, blocks);
+    if (e != hipSuccess) return fail(ctx, TDSE_ERR_HIP, std::string("tdse_copy_probe: ") + hipGetErrorString(e));
+    if (avg_ms) *avg_ms = ms / launches;
     return TDSE_OK;
 }
 

@@ -275,9 +275,14 @@ __device__ __forceinline__ void cn_cayley_solve(double2 (&x)[M], const int k, co
 // coef(i, hm, h, hp) as for cn_partition; sub = i hm, diag = 1 + i h, sup = i hp.
 // coef2 is called in pass 2 and must re-read the coefficients (not reuse
 // pass-1 values held in registers across the reduced solve).
-template <int M, class Coef, class Coef2>
+// EXPLICIT: the right-hand side is the explicit half (1 - i H~) x, formed
+// row by row in pass 1 from x and the chunk neighbours xl = x_{km-1},
+// xr = x_{km+m} (chunk_neighbours); otherwise it is x itself.
+template <int M, class Coef, class Coef2, bool EXPLICIT = false>
 __device__ __forceinline__ void cn_implicit_lowreg(double2 (&x)[M], const int k, const int P, Coef coef,
-                                                   Coef2 coef2, double2* __restrict__ lds) {
+                                                   Coef2 coef2, double2* __restrict__ lds,
+                                                   const double2 xl = make_double2(0.0, 0.0),
+                                                   const double2 xr = make_double2(0.0, 0.0)) {
     double2 ga[M - 1], de[M - 1];
     double2 al = cmk(0.0, 0.0), pi = cmk(1.0, 0.0), sa = cmk(0.0, 0.0), sd = cmk(0.0, 0.0);
     double2 gprev = cmk(0.0, 0.0), dprev = cmk(0.0, 0.0);
@@ -287,13 +292,22 @@ __device__ __forceinline__ void cn_implicit_lowreg(double2 (&x)[M], const int k,
     for (int i = 0; i < M; ++i) {
         double hm, h, hp;
         coef(i, hm, h, hp);
+        double2 rhs = x[i];
+        if constexpr (EXPLICIT) {
+            // (1 - i H~) x on row i: x_i - i (hm x_{i-1} + h x_i + hp x_{i+1})
+            const double2 xm = (i == 0) ? xl : x[i > 0 ? i - 1 : 0];
+            const double2 xp = (i + 1 < M) ? x[i + 1 < M ? i + 1 : i] : xr;
+            const double2 hx =
+                cmk(fma(hm, xm.x, fma(h, x[i].x, hp * xp.x)), fma(hm, xm.y, fma(h, x[i].y, hp * xp.y)));
+            rhs = cmk(x[i].x + hx.y, x[i].y - hx.x);
+        }
         if (i < M - 1) {
             // den = (1 + i h) - (i hm) gamma_{i-1}
             const double2 den = (i == 0) ? cmk(1.0, h) : cmk(fma(hm, gprev.y, 1.0), fma(-hm, gprev.x, h));
             const double2 r = crecip(den);
             const double2 rsub = cimul(hm, r);
             al = (i == 0) ? rsub : cneg(cmul(rsub, al));
-            const double2 d = (i == 0) ? cmul(r, x[0]) : csub(cmul(r, x[i]), cmul(rsub, dprev));
+            const double2 d = (i == 0) ? cmul(r, rhs) : csub(cmul(r, rhs), cmul(rsub, dprev));
             const double2 g = cimul(hp, r);
             sa = cadd(sa, cmul(pi, al));
             sd = cadd(sd, cmul(pi, d));
@@ -303,7 +317,7 @@ __device__ __forceinline__ void cn_implicit_lowreg(double2 (&x)[M], const int k,
             gprev = g;
             dprev = d;
         } else {
-            dlast = x[i];
+            dlast = rhs;
             diaglast = cmk(1.0, h);
             sublast = hm;
             suplast = hp;

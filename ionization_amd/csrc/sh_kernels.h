@@ -283,12 +283,17 @@ __global__ void __launch_bounds__(256) k_sh_observe(ShSim* __restrict__ sims, co
 // (1 + i H~) y = (1 - i H~) x along r for one l-line, H~ = tau_n H0 real
 // symmetric tridiagonal: diagonal s_n (T_j + l(l+1) U_j [+ corr0 at l = 0,
 // j = 0]), off-diagonals s_n A_{j-1}, s_n A_j (mesh_operators.py:889-928,
-// 244-269). One workgroup of P threads per line, M (= m) rows per thread;
-// the solve is cn_implicit_lowreg: z = (1 + i H~)^-1 x, y = 2 z - x. The
-// solver keeps two complex values per row between its passes and re-reads the
-// (L2-resident) coefficient tables instead of holding them, so WPE waves fit
-// per SIMD; that occupancy hides the HBM latency of the line loads.
-template <int M, int MAXT, int WPE>
+// 244-269). One workgroup of P threads per line, M (= m) rows per thread,
+// solved by cn_implicit_lowreg, which keeps two complex values per row between
+// its passes and re-reads the (L2-resident) coefficient tables instead of
+// holding them, so WPE waves fit per SIMD; that occupancy hides the HBM
+// latency of the line loads. Two algebraically equal forms:
+//  * EXPLICIT: the right-hand side (1 - i H~) x is formed inside the solver's
+//    first pass from x and the neighbouring chunks' edge rows; every line is
+//    read once and written once (faster for M >= 10);
+//  * Cayley: z = (1 + i H~)^-1 x, y = 2 z - x, with x re-read at the end
+//    (fewer live registers; faster for M <= 8).
+template <int M, int MAXT, int WPE, bool EXPLICIT>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_sh_rphase(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines) {
     extern __shared__ double2 lds[];  // 3 * P
@@ -329,11 +334,19 @@ k_sh_rphase(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_line
         h = sn * (fma(ll1, ld1(U2, i * P), ld1(T2, i * P)) + (i == 0 ? c0 : 0.0));
         hm = sn * ld1(AP2, i * P);
     };
-    cn_implicit_lowreg<M>(x, k, P, coef, coef2, lds);
+    if constexpr (EXPLICIT) {
+        double2 xl, xr;
+        chunk_neighbours<M>(x, k, P, lds, xl, xr, false);
+        cn_implicit_lowreg<M, decltype(coef), decltype(coef2), true>(x, k, P, coef, coef2, lds, xl, xr);
 #pragma unroll
-    for (int i = 0; i < M; ++i) {
-        const double2 x0 = ld2(gl, (size_t)i * P);
-        st2(gl, (size_t)i * P, cmk(fma(2.0, x[i].x, -x0.x), fma(2.0, x[i].y, -x0.y)));
+        for (int i = 0; i < M; ++i) st2(gl, (size_t)i * P, x[i]);
+    } else {
+        cn_implicit_lowreg<M>(x, k, P, coef, coef2, lds);
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const double2 x0 = ld2(gl, (size_t)i * P);
+            st2(gl, (size_t)i * P, cmk(fma(2.0, x[i].x, -x0.x), fma(2.0, x[i].y, -x0.y)));
+        }
     }
 }
 

@@ -342,6 +342,12 @@ class DeviceContext:
         self._check(self._lib.tdse_pool_usage(self._ctx, ctypes.byref(used), ctypes.byref(total)))
         return used.value, total.value
 
+    def copy_probe(self, n_bytes, launches=3):
+        """Average ms of a streaming copy of n_bytes inside the pool (needs 2 n_bytes free)."""
+        ms = ctypes.c_double()
+        self._check(self._lib.tdse_copy_probe(self._ctx, int(n_bytes), int(launches), ctypes.byref(ms)))
+        return ms.value
+
     # -- op level
     def tdma(self, sub, diag, sup, d):
         """x = T^-1 d (cy.tdma semantics); sub[i] couples x[i-1], sup[i] couples x[i+1]."""

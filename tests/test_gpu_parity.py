@@ -83,6 +83,22 @@ def test_sh_split_operator_matches_oracle(device_ctx, r_points, l_bound):
     assert abs((1 - bound_dev) - (1 - bound_orc)) <= OBS_ATOL
 
 
+@pytest.mark.parametrize("rows", [2, 4, 6, 8, 10, 12, 14, 16])
+def test_sh_every_rows_per_thread_matches_oracle(device_ctx, rows):
+    """Every R-phase instance (rows per thread m, block size, explicit or Cayley form) on a
+    700-point radial line: m = 2..16 gives P = 384, 192, 128, 128, 128, 64, 64, 64."""
+    case = sh_case(r_points=700, l_bound=6, t_half_as=20.0, rows=rows)
+    comp = case["compiled"]
+    print(f"SH rows={rows} P={comp.P} m={comp.m} steps={comp.n_steps}", flush=True)
+    assert comp.m == rows
+    res = _run_sh_device(device_ctx, comp)
+    g, norms, ips, lnorm = case["oracle"].run(case["g0"], case["times"], case["dt"], case["store_every"],
+                                             basis=case["basis"])
+    assert rel_l2(res.g, g) <= WF_RTOL
+    assert np.max(np.abs(res.norms - norms)) <= OBS_ATOL
+    assert np.max(np.abs(res.ips - ips)) <= OBS_ATOL
+
+
 def test_sh_store_every_and_sparse_data(device_ctx):
     case = sh_case(r_points=120, l_bound=10, t_half_as=60.0, store_every=7)
     res = _run_sh_device(device_ctx, case["compiled"])
@@ -128,6 +144,23 @@ def test_line_cn_matches_oracle(device_ctx, z_points):
     slot = device_ctx.add_line(comp)
     device_ctx.line_run(comp.n_steps)
     assert device_ctx.finished(1, slot)
+    g_dev, norms_dev, ips_dev = device_ctx.download_line(slot)
+    device_ctx.remove(1, slot)
+    g, norms, ips = case["oracle"].run(case["g0"], case["times"], case["store_every"], basis=case["basis"])
+    assert rel_l2(g_dev, g) <= WF_RTOL
+    assert np.max(np.abs(norms_dev - norms)) <= OBS_ATOL
+    assert np.max(np.abs(ips_dev - ips)) <= OBS_ATOL
+
+
+@pytest.mark.parametrize("rows", [2, 4, 6, 8, 10, 12, 14, 16])
+def test_line_every_rows_per_thread_matches_oracle(device_ctx, rows):
+    """Every LineMesh kernel instance (rows per thread m) on a 900-point line."""
+    case = line_case(z_points=900, n_steps=60, rows=rows)
+    comp = case["compiled"]
+    print(f"Line rows={rows} P={comp.P} m={comp.m}", flush=True)
+    assert comp.m == rows
+    slot = device_ctx.add_line(comp)
+    device_ctx.line_run(comp.n_steps)
     g_dev, norms_dev, ips_dev = device_ctx.download_line(slot)
     device_ctx.remove(1, slot)
     g, norms, ips = case["oracle"].run(case["g0"], case["times"], case["store_every"], basis=case["basis"])
