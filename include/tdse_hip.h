@@ -132,6 +132,56 @@ int tdse_line_add(tdse_ctx *ctx, const tdse_line_desc *desc, const double *table
                   const double *fs, const double *g0, const double *basis, int32_t *slot);
 
 /*
+ * CylindricalSliceMesh simulation, length gauge, AlternatingDirectionImplicit
+ * (ionization/mesh/meshes.py:509-607, mesh_operators.py:550-631,
+ * evolution_methods.py:45-77): per step the explicit rho half, implicit z,
+ * explicit z, implicit rho, then the mask, with the Coulomb potential and the
+ * interaction split half-and-half between the directions
+ * (mesh_operators.py:244-298). Device layout: g[rho][p], p < npz =
+ * z_chunks * z_rows, position p holding z index (p % z_chunks) * z_rows +
+ * p / z_chunks; rho padded to nrpad = rho_chunks * rho_rows.
+ * The rho-sweep shape (rho_rows, lanes, rho_chunks) is one of (8, 8, <= 128),
+ * (8, 4, <= 128), (4, 8, <= 128), (16, 4, <= 128), with rho_chunks * lanes a
+ * multiple of 64.
+ */
+typedef struct {
+    int32_t z_points, rho_points;
+    int32_t z_chunks, z_rows;      /* z-sweep: threads per z-line (multiple of 64), rows per thread */
+    int32_t rho_chunks, rho_rows;  /* rho-sweep: threads per rho-line, rows per thread */
+    int32_t lanes;                 /* rho-lines per rho-sweep workgroup */
+    int32_t n_steps, store_every, n_basis;
+    int32_t has_mask, reserved;
+    double z_prefactor;            /* -hbar^2 / (2 mu dz^2)   (mesh_operators.py:561) */
+    double rho_prefactor;          /* -hbar^2 / (2 mu drho^2) (mesh_operators.py:578-580) */
+    double coulomb;                /* k Q q: V = coulomb / r  (potentials/static.py:26-38) */
+    double test_charge;
+    double delta;                  /* inner-product weight dz drho (meshes.py:541) */
+    double mask_inner, mask_outer, mask_smoothness;  /* RadialCosineMask (potentials/masks.py:38-89) */
+} tdse_cyl_desc;
+
+/*
+ * Add one cylindrical simulation. zt: npz doubles (z of every position, 0 for
+ * padding); rt: nrpad doubles (rho, 0 for padding); roff: nrpad doubles
+ * (rho_prefactor * c(j + 1) between rho j and j + 1, 0 past the line,
+ * mesh_operators.py:582-592); fs: N doubles E(t_{n+1}); ts: N doubles
+ * tau_n = (t_{n+1} - t_n) / (2 hbar); g0 and basis: nrpad * npz complex each.
+ * Replaces MeshSpecification.to_sim() + the first
+ * AlternatingDirectionImplicit.evolve (sims.py:46-118, evolution_methods.py:48-77).
+ */
+int tdse_cyl_add(tdse_ctx *ctx, const tdse_cyl_desc *desc, const double *zt, const double *rt, const double *roff,
+                 const double *fs, const double *ts, const double *g0, const double *basis, int32_t *slot);
+
+/*
+ * hold != 0: every later step of this simulation ends at the exact reference
+ * g (the rho-sweep leaves the next step's explicit half to the next step), so
+ * tdse_download after any run sees g after a whole number of steps; needed by
+ * per-step callers of QuantumMesh.evolve, callbacks and checkpoints
+ * (meshes.py:251-257, sims.py:296-330). hold = 0 fuses that half into the
+ * previous sweep. No-op for SH and Line, whose states can always be made exact.
+ */
+int tdse_set_hold(tdse_ctx *ctx, int32_t kind, int32_t slot, int32_t hold);
+
+/*
  * Advance every active simulation by up to n_steps steps (MeshSimulation.run,
  * sims.py:289-321). use_graph != 0 replays one captured hipGraph per step.
  * Finished simulations stay resident until removed.
@@ -141,7 +191,8 @@ int tdse_run(tdse_ctx *ctx, int32_t n_steps, int32_t use_graph);
 /* Advance only the LineMesh simulations, up to n_steps inside one launch. */
 int tdse_line_run(tdse_ctx *ctx, int32_t n_steps);
 
-/* kind: 0 = SH, 1 = Line. step: steps taken (n_steps + 1 once finished). */
+/* kind: 0 = SH, 1 = Line, 2 = Cyl. step: steps taken (SH and Line: n_steps + 1
+ * once finished; Cyl: n_steps once finished). */
 int tdse_sim_state(tdse_ctx *ctx, int32_t kind, int32_t slot, int32_t *step, int32_t *n_steps, int32_t *n_data);
 
 /*
@@ -160,6 +211,8 @@ int tdse_download(tdse_ctx *ctx, int32_t kind, int32_t slot, double *g, double *
  * before tdse_download when a caller inspects g mid-run: the per-step
  * QuantumMesh.evolve / callback / checkpoint points of MeshSimulation.run
  * (ionization/mesh/meshes.py:251-257, sims.py:296-330). No-op for LineMesh.
+ * Cyl: succeeds when the state is exact (finished, or run with tdse_set_hold),
+ * else returns TDSE_ERR_STATE.
  */
 int tdse_materialize(tdse_ctx *ctx, int32_t kind, int32_t slot);
 
@@ -186,6 +239,8 @@ int tdse_last_run_ms(tdse_ctx *ctx, double *ms);
 int tdse_profile(tdse_ctx *ctx, int32_t enable);
 int tdse_profile_read(tdse_ctx *ctx, double *lphase_ms, int64_t *lphase_launches, double *rphase_ms,
                       int64_t *rphase_launches);
+/* Same accumulators by kernel: 0 = SH L-phase, 1 = SH R-phase, 2 = Cyl z-sweep, 3 = Cyl rho-sweep. */
+int tdse_profile_kernel(tdse_ctx *ctx, int32_t kernel, double *ms, int64_t *launches);
 
 #ifdef __cplusplus
 }

@@ -48,6 +48,31 @@ class LineDesc(ctypes.Structure):
     ]
 
 
+class CylDesc(ctypes.Structure):
+    _fields_ = [
+        ("z_points", ctypes.c_int32),
+        ("rho_points", ctypes.c_int32),
+        ("z_chunks", ctypes.c_int32),
+        ("z_rows", ctypes.c_int32),
+        ("rho_chunks", ctypes.c_int32),
+        ("rho_rows", ctypes.c_int32),
+        ("lanes", ctypes.c_int32),
+        ("n_steps", ctypes.c_int32),
+        ("store_every", ctypes.c_int32),
+        ("n_basis", ctypes.c_int32),
+        ("has_mask", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("z_prefactor", ctypes.c_double),
+        ("rho_prefactor", ctypes.c_double),
+        ("coulomb", ctypes.c_double),
+        ("test_charge", ctypes.c_double),
+        ("delta", ctypes.c_double),
+        ("mask_inner", ctypes.c_double),
+        ("mask_outer", ctypes.c_double),
+        ("mask_smoothness", ctypes.c_double),
+    ]
+
+
 # name -> (restype, argtypes); every name here is declared in include/tdse_hip.h
 SIGNATURES = {
     "tdse_abi_version": (ctypes.c_int, []),
@@ -70,6 +95,12 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.POINTER(LineDesc)] + [c_double_p] * 5 + [c_int32_p],
     ),
+    "tdse_cyl_add": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.POINTER(CylDesc)] + [c_double_p] * 7 + [c_int32_p],
+    ),
+    "tdse_set_hold": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "tdse_profile_kernel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_double_p, c_int64_p]),
     "tdse_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "tdse_line_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "tdse_sim_state": (

@@ -1,0 +1,328 @@
+// CylindricalSliceMesh, length gauge, alternating-direction implicit step on gfx950.
+//
+// One reference step n (evolution_methods.py:52-77 with the total Hamiltonian
+// of mesh_operators.py:282-298, then the mask of meshes.py:256-257) is
+//     g <- M . I_rho . E_z . I_z . E_rho . g
+// with E_d = 1 - i tau_n H_d (DotOperator) and I_d = (1 + i tau_n H_d)^-1
+// (TDMAOperator) along direction d. H_z and H_rho each carry the kinetic term
+// of their direction (mesh_operators.py:561-600) plus half of the Coulomb
+// potential and half of the length-gauge interaction -z q E(t_{n+1})
+// (mesh_operators.py:244-298, 602-628); tau_n = (t_{n+1} - t_n) / (2 hbar).
+//
+// Two HBM sweeps per step (64 B per mesh point):
+//   z-sweep   E_z I_z = 2 (1 + i tau H_z)^-1 - 1 along every z-line (the two
+//             halves share one matrix, so they collapse to the Cayley form);
+//   rho-sweep E_rho(n+1) . [observe] . M . I_rho(n) along every rho-line: the
+//             implicit half of step n, the mask, the observables of the exact
+//             g after n+1 steps, and the explicit half of step n+1.
+// A simulation held at an exact state (phase 1: its last rho-sweep skipped the
+// explicit half, at the end of the run or when the host asked for exact states)
+// gets that half from the pre-sweep at the start of its next step.
+//
+// Layout of one simulation in HBM: g[rho][p], rho in [0, nrpad), p in [0, npz),
+// npz = Pz * Mz. The z index of position p is jz = (p % Pz) * Mz + p / Pz, as
+// for the SphericalHarmonicMesh radial layout, so the z-sweep thread k holds
+// rows jz = k Mz .. k Mz + Mz - 1 of one rho-line at unit stride across the
+// wave. The rho-sweep puts LANES consecutive positions (LANES z-lines) side by
+// side in every wave: a wave reads 64 / LANES runs of LANES * 16 B. Padding
+// (jz >= nz, rho >= nr) is zero in g and gets zero coefficients.
+#pragma once
+
+#include "common.h"
+#include "line_solver.h"
+#include "sh_kernels.h"
+
+struct CylSim {
+    double2* g;             // [nrpad][npz]
+    const double* zt;       // [npz] z of every position (0 for padding)
+    const double* rt;       // [nrpad] rho of every rho index (0 for padding)
+    const double* roff;     // [nrpad] rho off-diagonal between j and j + 1: rp c(j + 1); 0 past the line
+    const double* fs;       // [N] E(t_{n+1})
+    const double* ts;       // [N] tau_n = (t_{n+1} - t_n) / (2 hbar)
+    double* part;           // [n_groups][1 + 2 n_basis] observation partials of the rho-sweep blocks
+    double* out_norm;       // [n_data]
+    double2* out_ip;        // [n_data][n_basis]
+    const double2* basis;   // [n_basis][nrpad][npz] test states in device layout
+    double zp, rp;          // kinetic prefactors -hbar^2 / (2 mu dz^2), -hbar^2 / (2 mu drho^2)
+    double kc;              // Coulomb numerator k Q q: V = kc / r
+    double q;               // test charge
+    double delta;           // inner-product weight dz drho
+    double m_inner, m_outer, m_inv_s;  // RadialCosineMask (inner, outer, 1 / smoothness)
+    int has_mask;
+    int nz, nr, Pz, Mz, npz, Cr, Mr, nrpad, lanes, n_groups;
+    int N, store_every, n_basis;
+    int step;   // steps already advanced; device-updated
+    int active;
+    int phase;  // 1: g is the reference g after `step` steps; 0: the explicit rho half of step `step` is applied
+    int hold;   // 1: end every step at an exact state (phase 1)
+};
+
+// Half the Coulomb potential plus half the interaction at (z, rho):
+// internal_hamiltonian adds V / 2, total_hamiltonian adds (-z q E) / 2.
+__device__ __forceinline__ double cyl_half_potential(const CylSim& S, double z, double rho, double E) {
+    const double r = sqrt(z * z + rho * rho);
+    return 0.5 * (S.kc / r) + 0.5 * (-z * S.q * E);
+}
+
+// RadialCosineMask.__call__ (potentials/masks.py:76-89) at r.
+__device__ __forceinline__ double cyl_mask(const CylSim& S, double z, double rho) {
+    const double r = sqrt(z * z + rho * rho);
+    if (r < S.m_inner) return 1.0;
+    if (r >= S.m_outer) return 0.0;
+    double sn, cs;
+    tdse_sincos(0.5 * 3.141592653589793 * (r - S.m_inner) / fabs(S.m_outer - S.m_inner), &sn, &cs);
+    return pow(fabs(cs), S.m_inv_s);
+}
+
+// ------------------------------------------------------------------ z-sweep
+//
+// One workgroup of Pz threads per (simulation, rho-line); Mz rows per thread.
+// (1 + i tau H_z) y = (1 - i tau H_z) x with H_z tridiagonal: diagonal
+// -2 zp + V/2 - z q E / 2, off-diagonals zp inside the line
+// (mesh_operators.py:561-575). EXPLICIT as for k_sh_rphase.
+template <int M, int MAXT, int WPE, bool EXPLICIT>
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+k_cyl_z(CylSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines) {
+    extern __shared__ double2 lds[];  // 3 * Pz
+    const int line = blockIdx.x;
+    if (line >= n_lines) return;
+    const int2 lt = lines[line];
+    CylSim& S = sims[lt.x];
+    const int n = S.step;
+    if (!S.active || n >= S.N) return;  // uniform across the block
+    const int P = S.Pz, nz = S.nz;
+    const int k = threadIdx.x;
+    const double tau = ld1(S.ts, n);
+    const double E = ld1(S.fs, n);
+    const double rho = ld1(S.rt, lt.y);
+    const double tzp = tau * S.zp;
+    const double tdiag = tau * (S.zp * -2.0);
+    double2* __restrict__ gl = S.g + (size_t)lt.y * S.npz + k;
+    const double* __restrict__ zt = S.zt + k;
+
+    double2 x[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) x[i] = ld2(gl, (size_t)i * P);
+    auto coef = [&](int i, double& hm, double& h, double& hp) {
+        const int j = k * M + i;
+        const bool live = j < nz;
+        const double z = ld1(zt, i * P);
+        h = live ? fma(tau, cyl_half_potential(S, z, rho, E), tdiag) : 0.0;
+        hm = (live && j >= 1) ? tzp : 0.0;
+        hp = (j + 1 < nz) ? tzp : 0.0;
+    };
+    auto coef2 = [&](int i, double& hm, double& h) {
+        const double* zt2 = zt;
+        asm volatile("" : "+v"(zt2));
+        const int j = k * M + i;
+        const bool live = j < nz;
+        const double z = ld1(zt2, i * P);
+        h = live ? fma(tau, cyl_half_potential(S, z, rho, E), tdiag) : 0.0;
+        hm = (live && j >= 1) ? tzp : 0.0;
+    };
+    if constexpr (EXPLICIT) {
+        double2 xl, xr;
+        chunk_neighbours<M>(x, k, P, lds, xl, xr, false);
+        cn_implicit_lowreg<M, decltype(coef), decltype(coef2), true>(x, k, P, coef, coef2, lds, xl, xr);
+#pragma unroll
+        for (int i = 0; i < M; ++i) st2(gl, (size_t)i * P, x[i]);
+    } else {
+        cn_implicit_lowreg<M>(x, k, P, coef, coef2, lds);
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const double2 x0 = ld2(gl, (size_t)i * P);
+            st2(gl, (size_t)i * P, cmk(fma(2.0, x[i].x, -x0.x), fma(2.0, x[i].y, -x0.y)));
+        }
+    }
+}
+
+// ---------------------------------------------------------------- rho-sweep
+//
+// One workgroup of LANES * Cr threads per (simulation, group of LANES
+// consecutive positions): thread t works on line ln = t % LANES (position
+// p = LANES * group + ln) and chunk c = t / LANES (rho rows c Mr .. c Mr + Mr - 1).
+// H_rho: diagonal -2 rp + V/2 - z q E / 2, off-diagonals rp c(j + 1)
+// (mesh_operators.py:578-600).
+//
+// MAIN (every step n < N):  implicit half of step n, mask, observables of the
+//     exact g after n + 1 steps at data times, explicit half of step n + 1
+//     unless n + 1 == N or the simulation holds exact states.
+// PRE (phase 1 only): observables of g at step 0, explicit half of step n.
+enum { CYL_PRE = 0, CYL_MAIN = 1 };
+
+template <int LANES>
+__device__ __forceinline__ double cyl_block_sum(double v, double* red, int t, int nthreads) {
+    v = wave_sum(v);
+    __syncthreads();
+    if ((t & (TDSE_WAVE - 1)) == 0) red[t / TDSE_WAVE] = v;
+    __syncthreads();
+    double s = 0.0;
+    const int nw = (nthreads + TDSE_WAVE - 1) / TDSE_WAVE;
+    for (int w = 0; w < nw; ++w) s += red[w];  // fixed order
+    return s;
+}
+
+template <int M, int LANES, int MODE, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_cyl_rho(CylSim* __restrict__ sims, const int2* __restrict__ groups,
+                                                  int n_groups) {
+    extern __shared__ double2 lds[];  // 3 * Cr * LANES double2, then 16 doubles for reductions
+    const int gi = blockIdx.x;
+    if (gi >= n_groups) return;
+    const int2 gt = groups[gi];
+    CylSim& S = sims[gt.x];
+    const int n = S.step;
+    if (!S.active || n >= S.N) return;
+    if (MODE == CYL_PRE && S.phase != 1) return;
+    const int t = threadIdx.x;
+    const int ln = t % LANES, c = t / LANES;
+    const int P = S.Cr, nr = S.nr, npz = S.npz;
+    const int p = gt.y * LANES + ln;
+    const int jz = (p % S.Pz) * S.Mz + p / S.Pz;
+    const bool line_live = jz < S.nz;
+    const double z = ld1(S.zt, p);
+    double2* __restrict__ gl = S.g + (size_t)(c * M) * npz + p;
+    double* red = reinterpret_cast<double*>(lds + 3 * P * LANES);
+
+    double2 x[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) x[i] = ld2(gl, (size_t)i * npz);
+
+    // rho-direction coefficients of row i (rho index j = c M + i) for (tau, E);
+    // rt / roff are passed in so a second pass can hand in opaque copies.
+    auto rho_coef = [&](const double* rt, const double* roff, double tau, double E, int i, double& hm, double& h,
+                        double& hp) {
+        const int j = c * M + i;
+        const bool live = line_live && j < nr;
+        const double rho = ld1(rt, j);
+        h = live ? fma(tau, cyl_half_potential(S, z, rho, E), tau * (S.rp * -2.0)) : 0.0;
+        hm = (live && j >= 1) ? tau * ld1(roff, j >= 1 ? j - 1 : 0) : 0.0;
+        hp = live ? tau * ld1(roff, j) : 0.0;
+    };
+
+    if constexpr (MODE == CYL_MAIN) {
+        const double tau = ld1(S.ts, n), E = ld1(S.fs, n);
+        auto coef = [&](int i, double& hm, double& h, double& hp) { rho_coef(S.rt, S.roff, tau, E, i, hm, h, hp); };
+        auto coef2 = [&](int i, double& hm, double& h) {
+            const double* rt2 = S.rt;
+            const double* roff2 = S.roff;
+            asm volatile("" : "+v"(rt2), "+v"(roff2));
+            double hp;
+            rho_coef(rt2, roff2, tau, E, i, hm, h, hp);
+        };
+        cn_implicit_lowreg<M, decltype(coef), decltype(coef2), false, LANES>(x, c, P, coef, coef2, lds,
+                                                                            cmk(0.0, 0.0), cmk(0.0, 0.0), ln);
+        if (S.has_mask) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                const int j = c * M + i;
+                const double mk = (line_live && j < nr) ? cyl_mask(S, z, ld1(S.rt, j)) : 0.0;
+                x[i] = cscale(mk, x[i]);
+            }
+        }
+    }
+
+    // observables of the exact g after n + 1 (MAIN) or 0 (PRE) steps
+    const int obs = (MODE == CYL_MAIN) ? n + 1 : 0;
+    if ((MODE == CYL_MAIN || n == 0) && sh_is_data_step(obs, S.N, S.store_every)) {
+        const int nb = S.n_basis;
+        double* part = S.part + (size_t)gt.y * (1 + 2 * nb);
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < M; ++i) acc += cabs2(x[i]);
+        const double tot = cyl_block_sum<LANES>(acc, red, t, blockDim.x);
+        if (t == 0) part[0] = tot;
+        for (int b = 0; b < nb; ++b) {
+            const double2* phi = S.basis + (size_t)b * S.nrpad * npz + (size_t)(c * M) * npz + p;
+            double re = 0.0, im = 0.0;
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                const double2 f = ld2(phi, (size_t)i * npz);  // conj(f) x
+                re += fma(f.x, x[i].x, f.y * x[i].y);
+                im += fma(f.x, x[i].y, -f.y * x[i].x);
+            }
+            const double sre = cyl_block_sum<LANES>(re, red, t, blockDim.x);
+            const double sim = cyl_block_sum<LANES>(im, red, t, blockDim.x);
+            if (t == 0) {
+                part[1 + 2 * b] = sre;
+                part[2 + 2 * b] = sim;
+            }
+        }
+    }
+
+    // explicit half of the next step (MAIN) or of step n (PRE)
+    const int ne = (MODE == CYL_MAIN) ? n + 1 : n;
+    const bool expl = (MODE == CYL_PRE) || (ne < S.N && !S.hold);
+    if (expl) {
+        const double tau = ld1(S.ts, ne), E = ld1(S.fs, ne);
+        double2 xl, xr;
+        chunk_neighbours<M, LANES>(x, c, P, lds, xl, xr, true, ln);
+        double2 prev = xl;
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            double hm, h, hp;
+            rho_coef(S.rt, S.roff, tau, E, i, hm, h, hp);
+            const double2 xp = (i + 1 < M) ? x[i + 1 < M ? i + 1 : i] : xr;
+            const double2 hx = cmk(fma(hm, prev.x, fma(h, x[i].x, hp * xp.x)), fma(hm, prev.y, fma(h, x[i].y, hp * xp.y)));
+            prev = x[i];
+            x[i] = cmk(x[i].x + hx.y, x[i].y - hx.x);  // x - i hx
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < M; ++i) st2(gl, (size_t)i * npz, x[i]);
+}
+
+// Reduce the rho-sweep partials of every simulation at a data time: job
+// (sim, -1) gives the norm, (sim, b) the inner product with test state b.
+// PRE: after the pre-sweep (data time 0); MAIN: after the main sweep (n + 1).
+__global__ void __launch_bounds__(256) k_cyl_observe(CylSim* __restrict__ sims, const int2* __restrict__ jobs,
+                                                     int n_jobs, int after_main) {
+    const int job = blockIdx.x;
+    if (job >= n_jobs) return;
+    const int2 jb = jobs[job];
+    CylSim& S = sims[jb.x];
+    const int n = S.step;
+    if (!S.active || n >= S.N) return;
+    int obs;
+    if (after_main) {
+        obs = n + 1;
+    } else {
+        if (n != 0 || S.phase != 1) return;
+        obs = 0;
+    }
+    if (!sh_is_data_step(obs, S.N, S.store_every)) return;
+    const int slot = sh_data_slot(obs, S.N, S.store_every);
+    const int stride = 1 + 2 * S.n_basis;
+    const int off = jb.y < 0 ? 0 : 1 + 2 * jb.y;
+    __shared__ double red_re[256];
+    __shared__ double red_im[256];
+    const int t = threadIdx.x;
+    double re = 0.0, im = 0.0;
+    for (int g = t; g < S.n_groups; g += blockDim.x) {
+        re += ld1(S.part, (size_t)g * stride + off);
+        if (jb.y >= 0) im += ld1(S.part, (size_t)g * stride + off + 1);
+    }
+    red_re[t] = re;
+    red_im[t] = im;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            red_re[t] += red_re[t + s];
+            red_im[t] += red_im[t + s];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        if (jb.y < 0) S.out_norm[slot] = red_re[0] * S.delta;
+        else S.out_ip[(size_t)slot * S.n_basis + jb.y] = cmk(red_re[0] * S.delta, red_im[0] * S.delta);
+    }
+}
+
+// After a full step: advance the step counter and record whether g is exact.
+__global__ void k_cyl_advance(CylSim* sims, int n_sims) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n_sims && sims[s].active && sims[s].step < sims[s].N) {
+        const int next = sims[s].step + 1;
+        sims[s].phase = (next < sims[s].N && !sims[s].hold) ? 0 : 1;
+        sims[s].step = next;
+    }
+}

@@ -13,6 +13,7 @@
 
 #include "../../include/tdse_hip.h"
 #include "common.h"
+#include "cyl_kernels.h"
 #include "line_kernels.h"
 #include "sh_kernels.h"
 #include "tdma_kernels.h"
@@ -110,6 +111,13 @@ struct LineHost {
     std::vector<void*> blocks;
 };
 
+struct CylHost {
+    bool live = false;
+    CylSim dev{};
+    int n_data = 0;
+    std::vector<void*> blocks;
+};
+
 // Device table with host-side growth.
 template <class T>
 struct DevVec {
@@ -122,7 +130,8 @@ struct Bucket {
     int M = 0;  // exact rows per chunk-thread
     int P = 0;
     bool imag = false;  // Line: complex time steps (imaginary-time relaxation)
-    DevVec<int2> lines;  // SH: (slot, l); Line: (slot, 0)
+    int lanes = 1;      // cylindrical rho-sweep: lines per workgroup
+    DevVec<int2> lines;  // SH: (slot, l); Line: (slot, 0); Cyl z: (slot, rho); Cyl rho: (slot, position group)
 };
 
 // l segments of at most kSegRows rows per L-phase wave, balanced in size.
@@ -143,8 +152,13 @@ struct tdse_ctx {
     std::string err;
     std::vector<ShHost> sh;
     std::vector<LineHost> line;
+    std::vector<CylHost> cyl;
     ShSim* d_sh = nullptr;
     LineSim* d_line = nullptr;
+    CylSim* d_cyl = nullptr;
+    DevVec<int2> cyl_jobs;
+    std::vector<Bucket> cyl_z_buckets;
+    std::vector<Bucket> cyl_r_buckets;
     double* d_cl = nullptr;
     DevVec<ShWave> waves;
     DevVec<int2> jobs;
@@ -158,9 +172,9 @@ struct tdse_ctx {
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
-    std::vector<std::pair<int, int>> ev_pairs;  // (kind 0 = L-phase, 1 = R-phase, start event index)
-    double prof_ms[2] = {0.0, 0.0};
-    int64_t prof_n[2] = {0, 0};
+    std::vector<std::pair<int, int>> ev_pairs;  // (kind, start event index); kinds: SH L-phase, SH R-phase, Cyl z, Cyl rho
+    double prof_ms[4] = {0.0, 0.0, 0.0, 0.0};
+    int64_t prof_n[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -225,9 +239,24 @@ int rebuild_tables(tdse_ctx* ctx) {
         if (!h.live) continue;
         lnb[{h.dev.m, h.dev.P + (h.complex_steps ? 4096 : 0)}].push_back(make_int2(s, 0));
     }
+    std::vector<int2> cjobs;
+    std::map<std::pair<int, int>, std::vector<int2>> czb;             // (rows, chunks)
+    std::map<std::pair<int, int>, std::vector<int2>> crb;             // (rows, chunks * 16 + lanes)
+    for (int s = 0; s < (int)ctx->cyl.size(); ++s) {
+        const CylHost& h = ctx->cyl[s];
+        if (!h.live) continue;
+        const CylSim& C = h.dev;
+        cjobs.push_back(make_int2(s, -1));
+        for (int b = 0; b < C.n_basis; ++b) cjobs.push_back(make_int2(s, b));
+        auto& vz = czb[{C.Mz, C.Pz}];
+        for (int r = 0; r < C.nr; ++r) vz.push_back(make_int2(s, r));
+        auto& vr = crb[{C.Mr, C.Cr * 16 + C.lanes}];
+        for (int g = 0; g < C.n_groups; ++g) vr.push_back(make_int2(s, g));
+    }
     int rc;
     if ((rc = devvec_upload(ctx, ctx->waves, waves))) return rc;
     if ((rc = devvec_upload(ctx, ctx->jobs, jobs))) return rc;
+    if ((rc = devvec_upload(ctx, ctx->cyl_jobs, cjobs))) return rc;
     auto fill = [&](std::vector<Bucket>& dst, std::map<std::pair<int, int>, std::vector<int2>>& src) -> int {
         for (auto& b : dst)
             if (b.lines.ptr) (void)hipFree(b.lines.ptr);
@@ -245,6 +274,13 @@ int rebuild_tables(tdse_ctx* ctx) {
     };
     if ((rc = fill(ctx->sh_buckets, shb))) return rc;
     if ((rc = fill(ctx->line_buckets, lnb))) return rc;
+    if ((rc = fill(ctx->cyl_z_buckets, czb))) return rc;
+    if ((rc = fill(ctx->cyl_r_buckets, crb))) return rc;
+    for (auto& b : ctx->cyl_r_buckets) {  // key second = chunks * 16 + lanes
+        const int key = b.P;
+        b.P = key / 16;
+        b.lanes = key % 16;
+    }
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     drop_graph(ctx);
     ctx->dirty = false;
@@ -318,6 +354,47 @@ void dispatch_line(tdse_ctx* ctx, const Bucket& b, int n_steps) {
     else dispatch_line_m<true>(ctx, b, n_steps);
 }
 
+template <int M, int MAXT, int WPE>
+void launch_cyl_z(tdse_ctx* ctx, const Bucket& b) {
+    const size_t lds = 3 * (size_t)b.P * sizeof(double2);
+    hipLaunchKernelGGL((k_cyl_z<M, MAXT, WPE, (M >= 10)>), dim3((unsigned)b.lines.count), dim3(b.P), lds, ctx->stream,
+                       ctx->d_cyl, b.lines.ptr, (int)b.lines.count);
+}
+
+void dispatch_cyl_z(tdse_ctx* ctx, const Bucket& b) {
+    switch (b.M) {
+        case 2: launch_cyl_z<2, 512, 4>(ctx, b); break;
+        case 4: launch_cyl_z<4, 512, 4>(ctx, b); break;
+        case 6: launch_cyl_z<6, 512, 4>(ctx, b); break;
+        case 8: launch_cyl_z<8, 512, 4>(ctx, b); break;
+        case 10: launch_cyl_z<10, 256, 3>(ctx, b); break;
+        case 12: launch_cyl_z<12, 256, 2>(ctx, b); break;
+        case 14: launch_cyl_z<14, 256, 2>(ctx, b); break;
+        default: launch_cyl_z<16, 256, 1>(ctx, b); break;
+    }
+}
+
+template <int M, int LANES, int MAXT>
+void launch_cyl_rho(tdse_ctx* ctx, const Bucket& b, int mode) {
+    const size_t lds = 3 * (size_t)b.P * LANES * sizeof(double2) + 16 * sizeof(double);
+    const dim3 grid((unsigned)b.lines.count), block((unsigned)(b.P * LANES));
+    if (mode == CYL_PRE)
+        hipLaunchKernelGGL((k_cyl_rho<M, LANES, CYL_PRE, MAXT>), grid, block, lds, ctx->stream, ctx->d_cyl,
+                           b.lines.ptr, (int)b.lines.count);
+    else
+        hipLaunchKernelGGL((k_cyl_rho<M, LANES, CYL_MAIN, MAXT>), grid, block, lds, ctx->stream, ctx->d_cyl,
+                           b.lines.ptr, (int)b.lines.count);
+}
+
+// rho-sweep shapes (rows, lines per workgroup, block limit); the layout chooser
+// of the host package picks one of these.
+void dispatch_cyl_rho(tdse_ctx* ctx, const Bucket& b, int mode) {
+    if (b.M == 8 && b.lanes == 8) launch_cyl_rho<8, 8, 1024>(ctx, b, mode);
+    else if (b.M == 8 && b.lanes == 4) launch_cyl_rho<8, 4, 512>(ctx, b, mode);
+    else if (b.M == 4 && b.lanes == 8) launch_cyl_rho<4, 8, 1024>(ctx, b, mode);
+    else launch_cyl_rho<16, 4, 512>(ctx, b, mode);
+}
+
 // Launch one full step of every simulation kind onto the context stream.
 int launch_step(tdse_ctx* ctx) {
     const int nw = (int)ctx->waves.count;
@@ -337,6 +414,24 @@ int launch_step(tdse_ctx* ctx) {
         hipLaunchKernelGGL(k_sh_advance, dim3((ns + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh, ns);
     }
     for (auto& b : ctx->line_buckets) dispatch_line(ctx, b, 1);
+    if (!ctx->cyl_r_buckets.empty()) {
+        const int nj = (int)ctx->cyl_jobs.count;
+        for (auto& b : ctx->cyl_r_buckets) dispatch_cyl_rho(ctx, b, CYL_PRE);
+        hipLaunchKernelGGL(k_cyl_observe, dim3(nj), dim3(256), 0, ctx->stream, ctx->d_cyl, ctx->cyl_jobs.ptr, nj, 0);
+        for (auto& b : ctx->cyl_z_buckets) {
+            prof_begin(ctx, 2);
+            dispatch_cyl_z(ctx, b);
+            prof_end(ctx);
+        }
+        for (auto& b : ctx->cyl_r_buckets) {
+            prof_begin(ctx, 3);
+            dispatch_cyl_rho(ctx, b, CYL_MAIN);
+            prof_end(ctx);
+        }
+        hipLaunchKernelGGL(k_cyl_observe, dim3(nj), dim3(256), 0, ctx->stream, ctx->d_cyl, ctx->cyl_jobs.ptr, nj, 1);
+        const int nc = (int)ctx->cyl.size();
+        hipLaunchKernelGGL(k_cyl_advance, dim3((nc + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_cyl, nc);
+    }
     HIP_TRY(ctx, hipGetLastError());
     return TDSE_OK;
 }
@@ -416,8 +511,10 @@ int tdse_ctx_create(int device, size_t pool_bytes, tdse_ctx** out) {
               hipMemcpy(ctx->d_cl, cl.data(), kMaxL * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
               hipMalloc(&ctx->d_sh, kSlotCapacity * sizeof(ShSim)) == hipSuccess &&
               hipMalloc(&ctx->d_line, kSlotCapacity * sizeof(LineSim)) == hipSuccess &&
+              hipMalloc(&ctx->d_cyl, kSlotCapacity * sizeof(CylSim)) == hipSuccess &&
               hipMemset(ctx->d_sh, 0, kSlotCapacity * sizeof(ShSim)) == hipSuccess &&
               hipMemset(ctx->d_line, 0, kSlotCapacity * sizeof(LineSim)) == hipSuccess &&
+              hipMemset(ctx->d_cyl, 0, kSlotCapacity * sizeof(CylSim)) == hipSuccess &&
               hipEventCreate(&ctx->ev_start) == hipSuccess && hipEventCreate(&ctx->ev_stop) == hipSuccess;
     if (!ok) {
         tdse_ctx_destroy(ctx);
@@ -436,6 +533,12 @@ int tdse_ctx_destroy(tdse_ctx* ctx) {
         if (b.lines.ptr) (void)hipFree(b.lines.ptr);
     for (auto& b : ctx->line_buckets)
         if (b.lines.ptr) (void)hipFree(b.lines.ptr);
+    for (auto& b : ctx->cyl_z_buckets)
+        if (b.lines.ptr) (void)hipFree(b.lines.ptr);
+    for (auto& b : ctx->cyl_r_buckets)
+        if (b.lines.ptr) (void)hipFree(b.lines.ptr);
+    if (ctx->cyl_jobs.ptr) (void)hipFree(ctx->cyl_jobs.ptr);
+    if (ctx->d_cyl) (void)hipFree(ctx->d_cyl);
     if (ctx->waves.ptr) (void)hipFree(ctx->waves.ptr);
     if (ctx->jobs.ptr) (void)hipFree(ctx->jobs.ptr);
     if (ctx->d_cl) (void)hipFree(ctx->d_cl);
@@ -703,6 +806,119 @@ int tdse_line_add(tdse_ctx* ctx, const tdse_line_desc* desc, const double* table
     return TDSE_OK;
 }
 
+int tdse_cyl_add(tdse_ctx* ctx, const tdse_cyl_desc* desc, const double* zt, const double* rt, const double* roff,
+                 const double* fs, const double* ts, const double* g0, const double* basis, int32_t* slot) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    if (!desc || !zt || !rt || !roff || !fs || !ts || !g0 || !slot)
+        return fail(ctx, TDSE_ERR_ARG, "tdse_cyl_add: NULL argument");
+    const tdse_cyl_desc& D = *desc;
+    if (D.z_points < 2 || D.rho_points < 2) return fail(ctx, TDSE_ERR_ARG, "tdse_cyl_add: bad mesh size");
+    const int Pz = D.z_chunks, Mz = D.z_rows, Cr = D.rho_chunks, Mr = D.rho_rows, lanes = D.lanes;
+    if (Pz < TDSE_WAVE || Pz % TDSE_WAVE) return fail(ctx, TDSE_ERR_ARG, "tdse_cyl_add: z_chunks must be a positive multiple of 64");
+    if (Mz < 2 || Mz > 16 || (Mz & 1)) return fail(ctx, TDSE_ERR_ARG, "tdse_cyl_add: z_rows must be even and in [2, 16]");
+    if ((Mz <= 8 && Pz > 512) || (Mz > 8 && Pz > 256)) return fail(ctx, TDSE_ERR_ARG, "tdse_cyl_add: too many z chunks for z rows");
+    if ((int64_t)Pz * Mz < D.z_points) return fail(ctx, TDSE_ERR_ARG, "tdse_cyl_add: z_chunks * z_rows < z_points");
+    const bool shape_ok = (Mr == 8 && lanes == 8 && Cr * 8 <= 1024) || (Mr == 8 && lanes == 4 && Cr * 4 <= 512) ||
+                          (Mr == 4 && lanes == 8 && Cr * 8 <= 1024) || (Mr == 16 && lanes == 4 && Cr * 4 <= 512);
+    if (!shape_ok || Cr < 1 || (Cr * lanes) % TDSE_WAVE)
+        return fail(ctx, TDSE_ERR_ARG, "tdse_cyl_add: unsupported rho layout (rows, lanes, chunks)");
+    if ((int64_t)Cr * Mr < D.rho_points) return fail(ctx, TDSE_ERR_ARG, "tdse_cyl_add: rho_chunks * rho_rows < rho_points");
+    if (D.n_steps < 1) return fail(ctx, TDSE_ERR_ARG, "tdse_cyl_add: n_steps must be >= 1");
+    if (D.store_every == 0 || D.store_every < -1) return fail(ctx, TDSE_ERR_ARG, "tdse_cyl_add: store_every must be >= 1 or -1");
+    if (D.n_basis < 0 || (D.n_basis > 0 && !basis)) return fail(ctx, TDSE_ERR_ARG, "tdse_cyl_add: basis missing");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int s = -1;
+    for (int i = 0; i < (int)ctx->cyl.size(); ++i)
+        if (!ctx->cyl[i].live) {
+            s = i;
+            break;
+        }
+    if (s < 0) {
+        if ((int)ctx->cyl.size() >= kSlotCapacity) return fail(ctx, TDSE_ERR_NOMEM, "no free simulation slot");
+        ctx->cyl.emplace_back();
+        s = (int)ctx->cyl.size() - 1;
+    }
+    CylHost& H = ctx->cyl[s];
+    H.blocks.clear();
+    CylSim& S = H.dev;
+    S = CylSim{};
+    S.nz = D.z_points;
+    S.nr = D.rho_points;
+    S.Pz = Pz;
+    S.Mz = Mz;
+    S.npz = Pz * Mz;
+    S.Cr = Cr;
+    S.Mr = Mr;
+    S.nrpad = Cr * Mr;
+    S.lanes = lanes;
+    S.n_groups = S.npz / lanes;
+    S.N = D.n_steps;
+    S.store_every = D.store_every;
+    S.n_basis = D.n_basis;
+    S.zp = D.z_prefactor;
+    S.rp = D.rho_prefactor;
+    S.kc = D.coulomb;
+    S.q = D.test_charge;
+    S.delta = D.delta;
+    S.has_mask = D.has_mask;
+    S.m_inner = D.mask_inner;
+    S.m_outer = D.mask_outer;
+    S.m_inv_s = D.mask_smoothness > 0 ? 1.0 / D.mask_smoothness : 1.0;
+    S.step = 0;
+    S.active = 1;
+    S.phase = 1;
+    S.hold = 0;
+    H.n_data = n_data_points(S.N, S.store_every);
+    const size_t pts = (size_t)S.nrpad * S.npz;
+    double2 *g, *oip, *bas = nullptr;
+    double *dzt, *drt, *droff, *dfs, *dts, *part, *on;
+    int rc = 0;
+    if ((rc = copy_to_device(ctx, H.blocks, reinterpret_cast<const double2*>(g0), pts, &g)) ||
+        (rc = copy_to_device(ctx, H.blocks, zt, (size_t)S.npz, &dzt)) ||
+        (rc = copy_to_device(ctx, H.blocks, rt, (size_t)S.nrpad, &drt)) ||
+        (rc = copy_to_device(ctx, H.blocks, roff, (size_t)S.nrpad, &droff)) ||
+        (rc = copy_to_device(ctx, H.blocks, fs, (size_t)S.N, &dfs)) ||
+        (rc = copy_to_device(ctx, H.blocks, ts, (size_t)S.N, &dts)) ||
+        (rc = copy_to_device<double>(ctx, H.blocks, nullptr, (size_t)S.n_groups * (1 + 2 * S.n_basis), &part)) ||
+        (rc = copy_to_device<double>(ctx, H.blocks, nullptr, (size_t)H.n_data, &on)) ||
+        (rc = copy_to_device<double2>(ctx, H.blocks, nullptr, (size_t)H.n_data * std::max(1, S.n_basis), &oip)) ||
+        (S.n_basis > 0 &&
+         (rc = copy_to_device(ctx, H.blocks, reinterpret_cast<const double2*>(basis), (size_t)S.n_basis * pts, &bas)))) {
+        (void)hipStreamSynchronize(ctx->stream);
+        free_blocks(ctx, H.blocks);
+        return rc;
+    }
+    S.g = g;
+    S.zt = dzt;
+    S.rt = drt;
+    S.roff = droff;
+    S.fs = dfs;
+    S.ts = dts;
+    S.part = part;
+    S.out_norm = on;
+    S.out_ip = oip;
+    S.basis = bas;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->d_cyl + s, &S, sizeof(CylSim), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // host buffers may be released after return
+    H.live = true;
+    ctx->dirty = true;
+    *slot = s;
+    return TDSE_OK;
+}
+
+int tdse_set_hold(tdse_ctx* ctx, int32_t kind, int32_t slot, int32_t hold) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    if (kind == 0 || kind == 1) return TDSE_OK;  // SH and Line states can be made exact at any step
+    if (kind != 2) return fail(ctx, TDSE_ERR_ARG, "unknown simulation kind");
+    if (slot < 0 || slot >= (int)ctx->cyl.size() || !ctx->cyl[slot].live) return fail(ctx, TDSE_ERR_ARG, "bad Cyl slot");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    const int h = hold ? 1 : 0;
+    ctx->cyl[slot].dev.hold = h;
+    HIP_TRY(ctx, hipMemcpy(reinterpret_cast<char*>(ctx->d_cyl + slot) + offsetof(CylSim, hold), &h, sizeof(int),
+                           hipMemcpyHostToDevice));
+    return TDSE_OK;
+}
+
 int tdse_run(tdse_ctx* ctx, int32_t n_steps, int32_t use_graph) {
     if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
     if (n_steps < 0) return fail(ctx, TDSE_ERR_ARG, "tdse_run: n_steps must be >= 0");
@@ -768,6 +984,12 @@ int tdse_sim_state(tdse_ctx* ctx, int32_t kind, int32_t slot, int32_t* step, int
                                hipMemcpyDeviceToHost));
         if (n_steps) *n_steps = ctx->line[slot].dev.N;
         if (n_data) *n_data = ctx->line[slot].n_data;
+    } else if (kind == 2) {
+        if (slot < 0 || slot >= (int)ctx->cyl.size() || !ctx->cyl[slot].live) return fail(ctx, TDSE_ERR_ARG, "bad Cyl slot");
+        HIP_TRY(ctx, hipMemcpy(&st, reinterpret_cast<char*>(ctx->d_cyl + slot) + offsetof(CylSim, step), sizeof(int),
+                               hipMemcpyDeviceToHost));
+        if (n_steps) *n_steps = ctx->cyl[slot].dev.N;
+        if (n_data) *n_data = ctx->cyl[slot].n_data;
     } else {
         return fail(ctx, TDSE_ERR_ARG, "unknown simulation kind");
     }
@@ -793,6 +1015,14 @@ int tdse_download(tdse_ctx* ctx, int32_t kind, int32_t slot, double* g, double* 
         const LineHost& H = ctx->line[slot];
         const LineSim& S = H.dev;
         if (g) HIP_TRY(ctx, hipMemcpy(g, S.g, (size_t)S.npad * sizeof(double2), hipMemcpyDeviceToHost));
+        if (norms) HIP_TRY(ctx, hipMemcpy(norms, S.out_norm, H.n_data * sizeof(double), hipMemcpyDeviceToHost));
+        if (ips && S.n_basis > 0)
+            HIP_TRY(ctx, hipMemcpy(ips, S.out_ip, (size_t)H.n_data * S.n_basis * sizeof(double2), hipMemcpyDeviceToHost));
+    } else if (kind == 2) {
+        if (slot < 0 || slot >= (int)ctx->cyl.size() || !ctx->cyl[slot].live) return fail(ctx, TDSE_ERR_ARG, "bad Cyl slot");
+        const CylHost& H = ctx->cyl[slot];
+        const CylSim& S = H.dev;
+        if (g) HIP_TRY(ctx, hipMemcpy(g, S.g, (size_t)S.nrpad * S.npz * sizeof(double2), hipMemcpyDeviceToHost));
         if (norms) HIP_TRY(ctx, hipMemcpy(norms, S.out_norm, H.n_data * sizeof(double), hipMemcpyDeviceToHost));
         if (ips && S.n_basis > 0)
             HIP_TRY(ctx, hipMemcpy(ips, S.out_ip, (size_t)H.n_data * S.n_basis * sizeof(double2), hipMemcpyDeviceToHost));
@@ -824,6 +1054,15 @@ int tdse_upload_state(tdse_ctx* ctx, int32_t kind, int32_t slot, const double* g
         HIP_TRY(ctx, hipMemcpy(S.g, g, (size_t)S.npad * sizeof(double2), hipMemcpyHostToDevice));
         HIP_TRY(ctx, hipMemcpy(reinterpret_cast<char*>(ctx->d_line + slot) + offsetof(LineSim, step), &step, sizeof(int),
                                hipMemcpyHostToDevice));
+    } else if (kind == 2) {
+        if (slot < 0 || slot >= (int)ctx->cyl.size() || !ctx->cyl[slot].live) return fail(ctx, TDSE_ERR_ARG, "bad Cyl slot");
+        const CylSim& S = ctx->cyl[slot].dev;
+        if (step < 0 || step > S.N) return fail(ctx, TDSE_ERR_ARG, "step out of range");
+        HIP_TRY(ctx, hipMemcpy(S.g, g, (size_t)S.nrpad * S.npz * sizeof(double2), hipMemcpyHostToDevice));
+        const int st[2] = {step, 1};  // the uploaded g is the reference g after `step` steps
+        char* base = reinterpret_cast<char*>(ctx->d_cyl + slot);
+        HIP_TRY(ctx, hipMemcpy(base + offsetof(CylSim, step), &st[0], sizeof(int), hipMemcpyHostToDevice));
+        HIP_TRY(ctx, hipMemcpy(base + offsetof(CylSim, phase), &st[1], sizeof(int), hipMemcpyHostToDevice));
     } else {
         return fail(ctx, TDSE_ERR_ARG, "unknown simulation kind");
     }
@@ -847,6 +1086,12 @@ int tdse_remove(tdse_ctx* ctx, int32_t kind, int32_t slot) {
                                sizeof(int), hipMemcpyHostToDevice));
         free_blocks(ctx, ctx->line[slot].blocks);
         ctx->line[slot].live = false;
+    } else if (kind == 2) {
+        if (slot < 0 || slot >= (int)ctx->cyl.size() || !ctx->cyl[slot].live) return fail(ctx, TDSE_ERR_ARG, "bad Cyl slot");
+        HIP_TRY(ctx, hipMemcpy(reinterpret_cast<char*>(ctx->d_cyl + slot) + offsetof(CylSim, active), &zero, sizeof(int),
+                               hipMemcpyHostToDevice));
+        free_blocks(ctx, ctx->cyl[slot].blocks);
+        ctx->cyl[slot].live = false;
     } else {
         return fail(ctx, TDSE_ERR_ARG, "unknown simulation kind");
     }
@@ -860,6 +1105,18 @@ int tdse_materialize(tdse_ctx* ctx, int32_t kind, int32_t slot) {
         if (slot < 0 || slot >= (int)ctx->line.size() || !ctx->line[slot].live)
             return fail(ctx, TDSE_ERR_ARG, "bad Line slot");
         return TDSE_OK;  // a LineMesh always holds the reference g between launches
+    }
+    if (kind == 2) {
+        if (slot < 0 || slot >= (int)ctx->cyl.size() || !ctx->cyl[slot].live) return fail(ctx, TDSE_ERR_ARG, "bad Cyl slot");
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        int st[2] = {0, 0};  // step, phase
+        char* base = reinterpret_cast<char*>(ctx->d_cyl + slot);
+        HIP_TRY(ctx, hipMemcpy(&st[0], base + offsetof(CylSim, step), sizeof(int), hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, hipMemcpy(&st[1], base + offsetof(CylSim, phase), sizeof(int), hipMemcpyDeviceToHost));
+        if (st[1] == 1) return TDSE_OK;
+        return fail(ctx, TDSE_ERR_STATE,
+                    "cylindrical state is between steps (explicit rho half applied); call tdse_set_hold before running "
+                    "to stop at exact states");
     }
     if (kind != 0) return fail(ctx, TDSE_ERR_ARG, "unknown simulation kind");
     if (slot < 0 || slot >= (int)ctx->sh.size() || !ctx->sh[slot].live) return fail(ctx, TDSE_ERR_ARG, "bad SH slot");
@@ -892,8 +1149,10 @@ int tdse_last_run_ms(tdse_ctx* ctx, double* ms) {
 int tdse_profile(tdse_ctx* ctx, int32_t enable) {
     if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
     ctx->profiling = enable != 0;
-    ctx->prof_ms[0] = ctx->prof_ms[1] = 0.0;
-    ctx->prof_n[0] = ctx->prof_n[1] = 0;
+    for (int i = 0; i < 4; ++i) {
+        ctx->prof_ms[i] = 0.0;
+        ctx->prof_n[i] = 0;
+    }
     return TDSE_OK;
 }
 
@@ -904,6 +1163,14 @@ int tdse_profile_read(tdse_ctx* ctx, double* lphase_ms, int64_t* lphase_launches
     if (lphase_launches) *lphase_launches = ctx->prof_n[0];
     if (rphase_ms) *rphase_ms = ctx->prof_ms[1];
     if (rphase_launches) *rphase_launches = ctx->prof_n[1];
+    return TDSE_OK;
+}
+
+int tdse_profile_kernel(tdse_ctx* ctx, int32_t kernel, double* ms, int64_t* launches) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    if (kernel < 0 || kernel > 3) return fail(ctx, TDSE_ERR_ARG, "kernel must be 0..3");
+    if (ms) *ms = ctx->prof_ms[kernel];
+    if (launches) *launches = ctx->prof_n[kernel];
     return TDSE_OK;
 }
 

@@ -84,36 +84,43 @@ __device__ __forceinline__ void reduced_pcr(double2 A, double2 C, double2 D, con
 // substitution then solves the eliminated equations s (2 t + 1) - 1 from their
 // already solved neighbours. About 2 P equation updates in total instead of
 // P log2 P, at the same number of barriers.
+//
+// LANES > 1: LANES independent lines share the block, equation k of line ln at
+// LDS index k * LANES + ln (each array of 3 holds P * LANES entries); all lines
+// have the same P and step through the levels together.
+template <int LANES = 1>
 __device__ __forceinline__ void reduced_cr(double2 A, double2 C, double2 D, const int k, const int P,
-                                           double2* __restrict__ lds, double2& Yk, double2& Ykm1) {
-    double2* pa = lds;
-    double2* pc = lds + P;
-    double2* pd = lds + 2 * P;
+                                           double2* __restrict__ lds, double2& Yk, double2& Ykm1,
+                                           const int ln = 0) {
+    double2* pa = lds + ln;
+    double2* pc = lds + P * LANES + ln;
+    double2* pd = lds + 2 * P * LANES + ln;
     __syncthreads();
-    pa[k] = A;
-    pc[k] = C;
-    pd[k] = D;
+    pa[k * LANES] = A;
+    pc[k * LANES] = C;
+    pd[k * LANES] = D;
     int s = 1;
     for (; s < P; s <<= 1) {
         __syncthreads();
         const int kk = 2 * s * (k + 1) - 1;
         if (kk < P) {
-            const double2 a0 = pa[kk], c0 = pc[kk], d0 = pd[kk];
-            const double2 am = pa[kk - s], cm = pc[kk - s], dm = pd[kk - s];
+            const int e = kk * LANES, em = (kk - s) * LANES, ep = (kk + s) * LANES;
+            const double2 a0 = pa[e], c0 = pc[e], d0 = pd[e];
+            const double2 am = pa[em], cm = pc[em], dm = pd[em];
             double2 apl = cmk(0.0, 0.0), cpl = cmk(0.0, 0.0), dpl = cmk(0.0, 0.0);
             if (kk + s < P) {
-                apl = pa[kk + s];
-                cpl = pc[kk + s];
-                dpl = pd[kk + s];
+                apl = pa[ep];
+                cpl = pc[ep];
+                dpl = pd[ep];
             }
             const double2 bn = csub(csub(cmk(1.0, 0.0), cmul(a0, cm)), cmul(c0, apl));
             const double2 an2 = cneg(cmul(a0, am));
             const double2 cn2 = cneg(cmul(c0, cpl));
             const double2 dn2 = csub(csub(d0, cmul(a0, dm)), cmul(c0, dpl));
             const double2 ib = crecip(bn);
-            pa[kk] = cmul(an2, ib);
-            pc[kk] = cmul(cn2, ib);
-            pd[kk] = cmul(dn2, ib);
+            pa[e] = cmul(an2, ib);
+            pc[e] = cmul(cn2, ib);
+            pd[e] = cmul(dn2, ib);
         }
     }
     // equations still standing have no neighbours in range: their d is their solution
@@ -121,14 +128,14 @@ __device__ __forceinline__ void reduced_cr(double2 A, double2 C, double2 D, cons
         __syncthreads();
         const int kk = s * (2 * k + 1) - 1;
         if (kk < P) {
-            const double2 ym = (kk - s >= 0) ? pd[kk - s] : cmk(0.0, 0.0);
-            const double2 yp = (kk + s < P) ? pd[kk + s] : cmk(0.0, 0.0);
-            pd[kk] = csub(csub(pd[kk], cmul(pa[kk], ym)), cmul(pc[kk], yp));
+            const double2 ym = (kk - s >= 0) ? pd[(kk - s) * LANES] : cmk(0.0, 0.0);
+            const double2 yp = (kk + s < P) ? pd[(kk + s) * LANES] : cmk(0.0, 0.0);
+            pd[kk * LANES] = csub(csub(pd[kk * LANES], cmul(pa[kk * LANES], ym)), cmul(pc[kk * LANES], yp));
         }
     }
     __syncthreads();
-    Yk = pd[k];
-    Ykm1 = (k > 0) ? pd[k - 1] : cmk(0.0, 0.0);
+    Yk = pd[k * LANES];
+    Ykm1 = (k > 0) ? pd[(k - 1) * LANES] : cmk(0.0, 0.0);
 }
 
 // Off-diagonal coefficients w*hm, w*hp: complex in general; for IMAG (w = i s)
@@ -278,11 +285,12 @@ __device__ __forceinline__ void cn_cayley_solve(double2 (&x)[M], const int k, co
 // EXPLICIT: the right-hand side is the explicit half (1 - i H~) x, formed
 // row by row in pass 1 from x and the chunk neighbours xl = x_{km-1},
 // xr = x_{km+m} (chunk_neighbours); otherwise it is x itself.
-template <int M, class Coef, class Coef2, bool EXPLICIT = false>
+// LANES / ln: several interleaved lines per block, as for reduced_cr.
+template <int M, class Coef, class Coef2, bool EXPLICIT = false, int LANES = 1>
 __device__ __forceinline__ void cn_implicit_lowreg(double2 (&x)[M], const int k, const int P, Coef coef,
                                                    Coef2 coef2, double2* __restrict__ lds,
                                                    const double2 xl = make_double2(0.0, 0.0),
-                                                   const double2 xr = make_double2(0.0, 0.0)) {
+                                                   const double2 xr = make_double2(0.0, 0.0), const int ln = 0) {
     double2 ga[M - 1], de[M - 1];
     double2 al = cmk(0.0, 0.0), pi = cmk(1.0, 0.0), sa = cmk(0.0, 0.0), sd = cmk(0.0, 0.0);
     double2 gprev = cmk(0.0, 0.0), dprev = cmk(0.0, 0.0);
@@ -325,15 +333,15 @@ __device__ __forceinline__ void cn_implicit_lowreg(double2 (&x)[M], const int k,
     }
     const double2 gp0 = cmul(pi, gprev);  // pi_{m-2} gamma_{m-2}
     __syncthreads();
-    lds[k] = sa;
-    lds[P + k] = gp0;
-    lds[2 * P + k] = sd;
+    lds[k * LANES + ln] = sa;
+    lds[(P + k) * LANES + ln] = gp0;
+    lds[(2 * P + k) * LANES + ln] = sd;
     __syncthreads();
     double2 na = cmk(0.0, 0.0), ng = cmk(0.0, 0.0), nd = cmk(0.0, 0.0);
     if (k + 1 < P) {
-        na = lds[k + 1];
-        ng = lds[P + k + 1];
-        nd = lds[2 * P + k + 1];
+        na = lds[(k + 1) * LANES + ln];
+        ng = lds[(P + k + 1) * LANES + ln];
+        nd = lds[(2 * P + k + 1) * LANES + ln];
     }
     // reduced equation of the chunk's last row: A Y_{k-1} + B Y_k + C Y_{k+1} = D
     double2 A = cneg(cimul(sublast, al));
@@ -347,7 +355,7 @@ __device__ __forceinline__ void cn_implicit_lowreg(double2 (&x)[M], const int k,
         D = cmul(D, ib);
     }
     double2 Lk, Yk;
-    reduced_cr(A, C, D, k, P, lds, Yk, Lk);
+    reduced_cr<LANES>(A, C, D, k, P, lds, Yk, Lk, ln);
     // pass 2: subtract alpha_i L_k
     double2 mu = cmk(0.0, 0.0);
 #pragma unroll
@@ -372,14 +380,14 @@ __device__ __forceinline__ void cn_implicit_lowreg(double2 (&x)[M], const int k,
 
 // Exchange the first / last rows of every chunk through LDS so each thread
 // knows x_{km-1} and x_{km+m}. lds: 2 P double2.
-template <int M>
+template <int M, int LANES = 1>
 __device__ __forceinline__ void chunk_neighbours(const double2 (&x)[M], const int k, const int P,
                                                  double2* __restrict__ lds, double2& xl, double2& xr,
-                                                 const bool sync_first) {
+                                                 const bool sync_first, const int ln = 0) {
     if (sync_first) __syncthreads();
-    lds[k] = x[0];
-    lds[P + k] = x[M - 1];
+    lds[k * LANES + ln] = x[0];
+    lds[(P + k) * LANES + ln] = x[M - 1];
     __syncthreads();
-    xl = (k > 0) ? lds[P + k - 1] : cmk(0.0, 0.0);
-    xr = (k + 1 < P) ? lds[k + 1] : cmk(0.0, 0.0);
+    xl = (k > 0) ? lds[(P + k - 1) * LANES + ln] : cmk(0.0, 0.0);
+    xr = (k + 1 < P) ? lds[(k + 1) * LANES + ln] : cmk(0.0, 0.0);
 }

@@ -27,6 +27,7 @@ from .exceptions import DeviceError
 
 KIND_SH = 0
 KIND_LINE = 1
+KIND_CYL = 2
 
 SH_NTAB = 6
 LINE_NTAB = 5
@@ -290,6 +291,150 @@ def compile_line(*, z_bound, z_points, potential, field: Callable, mask, times, 
     )
 
 
+# rho-sweep shapes the library instantiates: (rows, lines per workgroup), in order of preference
+CYL_RHO_SHAPES = ((8, 4), (4, 8), (16, 4), (8, 8))
+CYL_RHO_MAX_CHUNKS = 128
+
+
+def choose_cyl_layout(z_points: int, rho_points: int, z_rows: Optional[int] = None,
+                      rho_shape: Optional[Tuple[int, int]] = None) -> Tuple[int, int, int, int, int]:
+    """
+    (Pz, Mz, Cr, Mr, lanes) for a cylindrical mesh: the z-sweep layout as for an
+    SH radial line, and a rho-sweep shape of Cr chunks of Mr rows with `lanes`
+    z-lines per workgroup (Cr * lanes a multiple of 64, Cr <= 128); least rho
+    padding first, then the order of CYL_RHO_SHAPES.
+    """
+    Pz, Mz = choose_layout(z_points, "sh", z_rows)
+    best = None
+    shapes = [tuple(rho_shape)] if rho_shape is not None else list(CYL_RHO_SHAPES)
+    for pref, (mr, lanes) in enumerate(shapes):
+        if (mr, lanes) not in CYL_RHO_SHAPES:
+            raise DeviceError(f"unsupported rho-sweep shape {(mr, lanes)}")
+        step = 64 // lanes
+        cr = -(-rho_points // mr)
+        cr = -(-cr // step) * step
+        if cr > CYL_RHO_MAX_CHUNKS:
+            continue
+        key = (cr * mr - rho_points, pref)
+        if best is None or key < best[0]:
+            best = (key, cr, mr, lanes)
+    if best is None:
+        raise DeviceError(f"no device rho layout for {rho_points} rho points")
+    return Pz, Mz, best[1], best[2], best[3]
+
+
+def to_cyl_positions(g: np.ndarray, Pz: int, Mz: int, nrpad: int) -> np.ndarray:
+    """(..., nz, nr) natural order -> (..., nrpad, Pz Mz) device order, zero padding."""
+    g = np.asarray(g)
+    nz, nr = g.shape[-2:]
+    jz = position_index(nz, Pz, Mz)
+    live = jz < nz
+    out = np.zeros(g.shape[:-2] + (nrpad, Pz * Mz), dtype=g.dtype)
+    out[..., :nr, live] = np.swapaxes(g[..., jz[live], :], -1, -2)
+    return out
+
+
+def from_cyl_positions(a: np.ndarray, nz: int, nr: int, Pz: int, Mz: int) -> np.ndarray:
+    """Inverse of to_cyl_positions."""
+    jz = position_index(nz, Pz, Mz)
+    live = jz < nz
+    out = np.empty(a.shape[:-2] + (nz, nr), dtype=a.dtype)
+    out[..., jz[live], :] = np.swapaxes(a[..., :nr, live], -1, -2)
+    return out
+
+
+@dataclass
+class CompiledCyl:
+    """One CylindricalSliceMesh simulation in device table form."""
+
+    z_points: int
+    rho_points: int
+    Pz: int
+    Mz: int
+    Cr: int
+    Mr: int
+    lanes: int
+    n_steps: int
+    store_every: int
+    z_prefactor: float
+    rho_prefactor: float
+    coulomb: float
+    test_charge: float
+    delta: float
+    mask: Optional[Tuple[float, float, float]]
+    zt: np.ndarray  # (npz,)
+    rt: np.ndarray  # (nrpad,)
+    roff: np.ndarray  # (nrpad,)
+    fs: np.ndarray  # (N,)
+    ts: np.ndarray  # (N,)
+    g0: np.ndarray  # (nrpad, npz) complex
+    basis: np.ndarray  # (nb, nrpad, npz) complex
+
+    @property
+    def npz(self):
+        return self.Pz * self.Mz
+
+    @property
+    def nrpad(self):
+        return self.Cr * self.Mr
+
+    @property
+    def mesh_points(self):
+        return self.z_points * self.rho_points
+
+    @property
+    def footprint_bytes(self):
+        nd = data_count(self.n_steps, self.store_every)
+        pts = self.npz * self.nrpad
+        nb = len(self.basis)
+        return int(16 * pts * (1 + nb) + 8 * (self.npz + 2 * self.nrpad) + 16 * self.n_steps
+                   + 8 * (self.npz // self.lanes) * (1 + 2 * nb) + nd * 16 * (1 + max(1, nb)) + 256 * 12)
+
+
+def compile_cyl(*, z_bound, rho_bound, z_points, rho_points, field: Callable, times, g0,
+                basis: Sequence[np.ndarray] = (), store_every=1, charge=u.proton_charge,
+                test_mass=u.electron_mass_reduced, test_charge=u.electron_charge, mask=None,
+                z_rows=None, rho_shape=None) -> CompiledCyl:
+    """
+    Compile a CylindricalSliceMesh + CylindricalSliceLengthGaugeOperators + ADI
+    simulation (meshes.py:509-607, mesh_operators.py:550-631,
+    evolution_methods.py:45-77) with the Coulomb potential of a point charge
+    ``charge`` (potentials/static.py:12-38). ``mask`` is None or the
+    (inner_radius, outer_radius, smoothness) of a RadialCosineMask
+    (potentials/masks.py:38-89). g0 and basis are (z_points, rho_points).
+    """
+    Pz, Mz, Cr, Mr, lanes = choose_cyl_layout(z_points, rho_points, z_rows, rho_shape)
+    nrpad = Cr * Mr
+    z = np.linspace(-z_bound, z_bound, z_points)
+    rho = np.delete(np.linspace(0, rho_bound, rho_points + 1), 0)
+    delta_z = z[1] - z[0]
+    delta_rho = rho[1] - rho[0]
+    rho = rho - delta_rho / 2
+    zp = -(u.hbar ** 2) / (2 * test_mass * (delta_z ** 2))
+    rp = -(u.hbar ** 2) / (2 * test_mass * (delta_rho ** 2))
+    jz = position_index(z_points, Pz, Mz)
+    zt = np.where(jz < z_points, z[np.minimum(jz, z_points - 1)], 0.0)
+    rt = np.zeros(nrpad)
+    rt[:rho_points] = rho
+    j = np.arange(1, rho_points, dtype=np.float64)  # c(j) couples rho indices j - 1 and j
+    roff = np.zeros(nrpad)
+    roff[: rho_points - 1] = (j / np.sqrt((j ** 2) - 0.25)) * rp
+    times = np.asarray(times, dtype=np.float64)
+    steps = np.diff(times)
+    fs = np.asarray(field(times[1:]), dtype=np.float64) * np.ones(len(steps))
+    ts = steps / (2 * u.hbar)
+    bmat = np.array([np.asarray(b, dtype=np.complex128) for b in basis]).reshape(len(basis), z_points, rho_points)
+    return CompiledCyl(
+        z_points=int(z_points), rho_points=int(rho_points), Pz=Pz, Mz=Mz, Cr=Cr, Mr=Mr, lanes=lanes,
+        n_steps=len(steps), store_every=int(store_every), z_prefactor=float(zp), rho_prefactor=float(rp),
+        coulomb=float(u.coulomb_constant * charge * test_charge), test_charge=float(test_charge),
+        delta=float(delta_z * delta_rho), mask=None if mask is None else tuple(float(v) for v in mask),
+        zt=np.ascontiguousarray(zt), rt=rt, roff=roff, fs=np.ascontiguousarray(fs), ts=np.ascontiguousarray(ts),
+        g0=np.ascontiguousarray(to_cyl_positions(np.asarray(g0, np.complex128), Pz, Mz, nrpad)),
+        basis=np.ascontiguousarray(to_cyl_positions(bmat, Pz, Mz, nrpad)),
+    )
+
+
 # ----------------------------------------------------------------- context
 
 
@@ -312,6 +457,7 @@ class DeviceContext:
         self.pool_bytes = int(pool_bytes)
         self._sh = {}
         self._line = {}
+        self._cyl = {}
 
     # -- lifetime
     def close(self):
@@ -391,6 +537,35 @@ class DeviceContext:
         self._line[slot.value] = c
         return slot.value
 
+    def add_cyl(self, c: CompiledCyl) -> int:
+        mk = c.mask if c.mask is not None else (0.0, 0.0, 1.0)
+        desc = _lib.CylDesc(c.z_points, c.rho_points, c.Pz, c.Mz, c.Cr, c.Mr, c.lanes, c.n_steps, c.store_every,
+                            len(c.basis), 0 if c.mask is None else 1, 0, c.z_prefactor, c.rho_prefactor, c.coulomb,
+                            c.test_charge, c.delta, mk[0], mk[1], mk[2])
+        slot = ctypes.c_int32()
+        basis = np.ascontiguousarray(c.basis, dtype=np.complex128) if len(c.basis) else None
+        self._check(self._lib.tdse_cyl_add(
+            self._ctx, ctypes.byref(desc), _lib.dptr(c.zt), _lib.dptr(c.rt), _lib.dptr(c.roff), _lib.dptr(c.fs),
+            _lib.dptr(c.ts), _lib.dptr(c.g0), _lib.dptr(basis), ctypes.byref(slot)))
+        self._cyl[slot.value] = c
+        return slot.value
+
+    def set_hold(self, kind: int, slot: int, hold: bool):
+        """Cyl: end every later step at the exact reference g (see tdse_set_hold)."""
+        self._check(self._lib.tdse_set_hold(self._ctx, kind, slot, 1 if hold else 0))
+
+    def download_cyl(self, slot: int, with_g: bool = True):
+        """(g (z_points, rho_points), norms, inner products) of a cylindrical simulation."""
+        c = self._cyl[slot]
+        _, _, nd = self.state(KIND_CYL, slot)
+        g = np.empty((c.nrpad, c.npz), dtype=np.complex128) if with_g else None
+        norms = np.empty(nd)
+        ips = np.empty((nd, max(1, len(c.basis))), dtype=np.complex128)
+        self._check(self._lib.tdse_download(self._ctx, KIND_CYL, slot, _lib.dptr(g), _lib.dptr(norms), None,
+                                            _lib.dptr(ips)))
+        gn = from_cyl_positions(g, c.z_points, c.rho_points, c.Pz, c.Mz) if with_g else None
+        return gn, norms, ips[:, : len(c.basis)]
+
     def run(self, n_steps: int, use_graph: bool = True) -> float:
         """Advance every simulation by n_steps steps; returns the device time in ms."""
         self._check(self._lib.tdse_run(self._ctx, int(n_steps), 1 if use_graph else 0))
@@ -413,7 +588,7 @@ class DeviceContext:
 
     def finished(self, kind: int, slot: int) -> bool:
         st, nn, _ = self.state(kind, slot)
-        return st > nn
+        return st >= nn if kind == KIND_CYL else st > nn
 
     def download_sh(self, slot: int, with_g: bool = True) -> ShResult:
         c = self._sh[slot]
@@ -438,8 +613,12 @@ class DeviceContext:
         return from_positions(g, c.z_points, c.P, c.m), norms, ips[:, : len(c.basis)]
 
     def upload_state(self, kind: int, slot: int, g_natural: np.ndarray, step: int):
-        c = self._sh[slot] if kind == KIND_SH else self._line[slot]
-        gp = np.ascontiguousarray(to_positions(np.asarray(g_natural, np.complex128), c.P, c.m))
+        if kind == KIND_CYL:
+            c = self._cyl[slot]
+            gp = np.ascontiguousarray(to_cyl_positions(np.asarray(g_natural, np.complex128), c.Pz, c.Mz, c.nrpad))
+        else:
+            c = self._sh[slot] if kind == KIND_SH else self._line[slot]
+            gp = np.ascontiguousarray(to_positions(np.asarray(g_natural, np.complex128), c.P, c.m))
         self._check(self._lib.tdse_upload_state(self._ctx, kind, slot, _lib.dptr(gp), int(step)))
 
     def materialize(self, kind: int, slot: int):
@@ -448,7 +627,7 @@ class DeviceContext:
 
     def remove(self, kind: int, slot: int):
         self._check(self._lib.tdse_remove(self._ctx, kind, slot))
-        (self._sh if kind == KIND_SH else self._line).pop(slot, None)
+        {KIND_SH: self._sh, KIND_LINE: self._line, KIND_CYL: self._cyl}[kind].pop(slot, None)
 
     def profile(self, enable: bool):
         self._check(self._lib.tdse_profile(self._ctx, 1 if enable else 0))
@@ -460,3 +639,9 @@ class DeviceContext:
                                                 ctypes.byref(rn)))
         return {"lphase_ms": lm.value, "lphase_launches": ln.value, "rphase_ms": rm.value,
                 "rphase_launches": rn.value}
+
+    def profile_kernel(self, kernel: int):
+        """(ms, launches) accumulated for kernel 0 = SH L-phase, 1 = SH R-phase, 2 = Cyl z, 3 = Cyl rho."""
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        self._check(self._lib.tdse_profile_kernel(self._ctx, int(kernel), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

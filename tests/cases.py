@@ -66,3 +66,34 @@ def line_case(*, z_bound_bohr=50.0, z_points=500, n_steps=300, e0_au=0.05, omega
                                times=times, g0=g0, basis=basis, store_every=store_every, test_mass=mass, rows=rows)
     return dict(oracle=orc, compiled=comp, g0=g0.astype(np.complex128), times=times, basis=basis,
                 store_every=store_every)
+
+
+def cyl_case(*, z_bound_bohr=20.0, rho_bound_bohr=10.0, z_points=120, rho_points=60, pulse_width_as=50.0,
+             fluence_jcm2=5.0, t_half_as=40.0, dt_as=1.0, store_every=1, mask=True, z_rows=None, rho_shape=None):
+    """CylindricalSliceMesh ADI with a sinc pulse, a radial absorber and a hydrogen 1s start."""
+    zb = z_bound_bohr * u.bohr_radius
+    rb = rho_bound_bohr * u.bohr_radius
+    pw = pulse_width_as * u.asec
+    F = fluence_jcm2 * u.Jcm2
+    field = lambda t: physics.sinc_pulse_field(t, pulse_width=pw, fluence=F, phase=0.0)
+    mask_params = (0.7 * zb, zb, 8.0) if mask else None
+    maskf = (lambda r: physics.radial_cosine_mask(r, mask_params[0], mask_params[1])) if mask else None
+    orc = mesh_cpu.CylOracle(z_bound=zb, rho_bound=rb, z_points=z_points, rho_points=rho_points,
+                             potential=physics.coulomb, field=field, mask=maskf)
+    w = orc.delta_z * orc.delta_rho
+    gf = np.sqrt(u.twopi * orc.rho_mesh)
+
+    def normed(g):
+        g = g.astype(np.complex128)
+        return g / np.sqrt(np.sum(np.abs(g) ** 2) * w)
+
+    a0 = u.bohr_radius
+    g0 = normed(gf * np.exp(-orc.r_mesh / a0))                                   # 1s
+    basis = [g0, normed(gf * orc.z_mesh * np.exp(-orc.r_mesh / (2 * a0))),        # 2p0-like
+             normed(gf * (2 - orc.r_mesh / a0) * np.exp(-orc.r_mesh / (2 * a0)))]  # 2s-like
+    dt = dt_as * u.asec
+    times = mesh_cpu.time_grid(-t_half_as * u.asec, t_half_as * u.asec, dt)
+    comp = engine.compile_cyl(z_bound=zb, rho_bound=rb, z_points=z_points, rho_points=rho_points, field=field,
+                              times=times, g0=g0, basis=basis, store_every=store_every, mask=mask_params,
+                              z_rows=z_rows, rho_shape=rho_shape)
+    return dict(oracle=orc, compiled=comp, g0=g0, times=times, basis=basis, store_every=store_every)
