@@ -93,6 +93,45 @@ def time_plain(steps, warmup):
     return dict(points=comp.mesh_points, wall_s=wall)
 
 
+def time_cyl(n_sims, seed, steps, warmup):
+    """The batched CylindricalSliceMesh ADI workload: one graph replay advances all simulations."""
+    from ionization_amd import workloads
+    from ionization_amd.engine import DeviceContext
+
+    compiled = workloads.cyl_scan_compiled(workloads.cyl_scan_params(n_sims, seed))
+    pool = int(sum(c.footprint_bytes for c in compiled) * 1.1) + (256 << 20)
+    ctx = DeviceContext(pool_bytes=pool)
+    for c in compiled:
+        ctx.add_cyl(c)
+    points = sum(c.mesh_points for c in compiled)
+    padded = sum(c.npz * c.nrpad for c in compiled)
+    ctx.run(warmup)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.run(steps, use_graph=True)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ctx.profile(True)
+    ctx.run(steps, use_graph=False)
+    zk = ctx.profile_kernel(2)
+    rk = ctx.profile_kernel(3)
+    ctx.profile(False)
+    ctx.close()
+    rate = points * steps / wall
+    out = {"workload": f"cyl_scan: {n_sims} CylindricalSliceMesh simulations (z 2000 x rho 1000), seed {seed}, "
+                       "ADI length gauge, dt = 1 as, one hipGraph replay per step",
+           "value": rate, "unit": "mesh-point-steps/s", "ms_per_step": wall * 1e3 / steps,
+           "mesh_points": points, "step_hbm_gbs": rate * BYTES_PER_POINT_STEP / 1e9, "kernels": {}}
+    for name, (ms, n) in (("k_cyl_z", zk), ("k_cyl_rho", rk)):
+        if n:
+            avg = ms / n
+            by = BYTES_PER_POINT_PHASE * points * steps / n
+            out["kernels"][name] = {"avg_ms": avg, "algorithmic_bytes": by, "achieved_gbs": by / (avg * 1e-3) / 1e9,
+                                    "padded_points_frac": padded / points}
+    log(f"cyl: {out['ms_per_step']:.3f} ms/step, {rate / 1e9:.2f} G mesh-point-steps/s")
+    return out
+
+
 def cpu_baseline(params, seconds):
     """The oracle on one core: one simulation of the scan, as many steps as fit in about `seconds`."""
     from ionization_amd import units as u
@@ -250,6 +289,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--cyl-sims", type=int, default=32, help="0 skips the cylindrical ADI workload")
+    ap.add_argument("--cyl-steps", type=int, default=100)
     ap.add_argument("--summarize-profiles", metavar="DIR")
     ap.add_argument("--trace-command", default="")
     ap.add_argument("--fetch-command", default="")
@@ -292,6 +333,8 @@ def main():
     measured = load_traffic().get(dom_name)
     traffic = None if measured is None else measured[0] + measured[1]
     step_gbs = rate * BYTES_PER_POINT_STEP / 1e9
+
+    cyl = time_cyl(args.cyl_sims, args.seed, args.cyl_steps, 10) if args.cyl_sims > 0 else None
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -337,6 +380,7 @@ def main():
             "traffic": traffic,
         },
         "cpu_baseline": cpu,
+        "cyl_adi": cyl,
     }
     print(json.dumps(out), flush=True)
     if args.calibrate_copy:  # after the JSON line: a known-size kernel for the counter passes

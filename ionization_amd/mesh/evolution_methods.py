@@ -11,7 +11,7 @@ step through the same kernels.
 
 import numpy as np
 
-from .. import core, engine
+from .. import core, engine, potentials
 from .. import units as u
 from . import meshes
 
@@ -86,7 +86,8 @@ class AlternatingDirectionImplicit(EvolutionMethod):
     """
     Crank-Nicolson / ADI step (evolution_methods.py:46-77). On a LineMesh the
     total Hamiltonian is one operator, so a step is (1 + i tau H)^-1 (1 - i tau H)
-    followed by the mask.
+    followed by the mask; on a CylindricalSliceMesh it is explicit rho, implicit
+    z, explicit z, implicit rho, then the mask.
     """
 
     def compile(self, sim, g0=None):
@@ -103,4 +104,34 @@ class AlternatingDirectionImplicit(EvolutionMethod):
                 store_every=sim.spec.store_data_every, test_mass=sim.spec.test_mass,
                 test_charge=sim.spec.test_charge, rows=getattr(sim.spec, "device_rows", None),
             )
+        if isinstance(mesh, meshes.CylindricalSliceMesh):
+            if sim.spec.operators.gauge != core.Gauge.LENGTH:
+                raise NotImplementedError("the device CylindricalSliceMesh step implements the length gauge")
+            basis = [np.asarray(mesh.get_g_for_state(s), dtype=np.complex128) for s in sim.spec.test_states]
+            return engine.compile_cyl(
+                z_bound=sim.spec.z_bound, rho_bound=sim.spec.rho_bound, z_points=sim.spec.z_points,
+                rho_points=sim.spec.rho_points, field=_field(sim), times=sim.times,
+                g0=mesh.g if g0 is None else g0, basis=basis, store_every=sim.spec.store_data_every,
+                charge=_coulomb_charge(sim.spec.internal_potential), test_mass=sim.spec.test_mass,
+                test_charge=sim.spec.test_charge, mask=_radial_mask(sim.spec.mask),
+            )
         raise NotImplementedError(f"device ADI is not implemented for {mesh.__class__.__name__}")
+
+
+def _coulomb_charge(pot):
+    """The point charge of a Coulomb internal potential: the cylindrical kernels evaluate
+    k Q q / r on the fly (potentials/static.py:12-38)."""
+    if isinstance(pot, potentials.PotentialEnergySum) and len(pot._container) == 1:
+        pot = pot._container[0]
+    if isinstance(pot, potentials.CoulombPotential):
+        return pot.charge
+    raise NotImplementedError("the device CylindricalSliceMesh step supports a CoulombPotential internal potential")
+
+
+def _radial_mask(mask):
+    """(inner, outer, smoothness) of a RadialCosineMask, None for NoMask."""
+    if isinstance(mask, potentials.NoMask):
+        return None
+    if isinstance(mask, potentials.RadialCosineMask):
+        return (mask.inner_radius, mask.outer_radius, mask.smoothness)
+    raise NotImplementedError("the device CylindricalSliceMesh step supports NoMask and RadialCosineMask")

@@ -35,15 +35,22 @@ class _DeviceSession:
     def __init__(self, sim, device=0):
         method = sim.spec.evolution_method
         self.compiled = method.compile(sim)
-        self.kind = engine.KIND_SH if isinstance(self.compiled, engine.CompiledSH) else engine.KIND_LINE
+        if isinstance(self.compiled, engine.CompiledSH):
+            self.kind = engine.KIND_SH
+        elif isinstance(self.compiled, engine.CompiledCyl):
+            self.kind = engine.KIND_CYL
+        else:
+            self.kind = engine.KIND_LINE
         pool = int(self.compiled.footprint_bytes * 1.25) + (64 << 20)
         self.ctx = engine.DeviceContext(device=device, pool_bytes=pool)
-        if self.kind == engine.KIND_SH:
-            self.slot = self.ctx.add_sh(self.compiled)
-        else:
-            self.slot = self.ctx.add_line(self.compiled)
+        add = {engine.KIND_SH: self.ctx.add_sh, engine.KIND_LINE: self.ctx.add_line, engine.KIND_CYL: self.ctx.add_cyl}
+        self.slot = add[self.kind](self.compiled)
         if sim.time_index > 0:
             self.ctx.upload_state(self.kind, self.slot, sim.mesh.g, sim.time_index)
+
+    def hold(self, on: bool):
+        """End later steps at exact states (cylindrical meshes; a no-op for the others)."""
+        self.ctx.set_hold(self.kind, self.slot, on)
 
     def step_index(self):
         return self.ctx.state(self.kind, self.slot)[0]
@@ -52,12 +59,17 @@ class _DeviceSession:
         self.ctx.materialize(self.kind, self.slot)
         if self.kind == engine.KIND_SH:
             return self.ctx.download_sh(self.slot, with_g=True).g
+        if self.kind == engine.KIND_CYL:
+            return self.ctx.download_cyl(self.slot)[0]
         return self.ctx.download_line(self.slot)[0]
 
     def data(self):
         if self.kind == engine.KIND_SH:
             res = self.ctx.download_sh(self.slot, with_g=False)
             return {"norms": res.norms, "lnorms": res.lnorms, "ips": res.ips}
+        if self.kind == engine.KIND_CYL:
+            _, norms, ips = self.ctx.download_cyl(self.slot, with_g=False)
+            return {"norms": norms, "lnorms": None, "ips": ips}
         _, norms, ips = self.ctx.download_line(self.slot)
         return {"norms": norms, "lnorms": None, "ips": ips}
 
@@ -151,6 +163,7 @@ class MeshSimulation:
         """One reference step at the current time index (QuantumMesh.evolve's seam)."""
         sess = self._session_open()
         n = self.time_index - 1 if self.time_index > 0 else 0
+        sess.hold(True)
         sess.ctx.upload_state(sess.kind, sess.slot, self.mesh.g, n)
         sess.ctx.run(1, use_graph=False)
         self.mesh.g = sess.g()
@@ -179,6 +192,7 @@ class MeshSimulation:
         try:
             if callback is not None:
                 # per-step: the reference exposes g after every step to the callback
+                sess.hold(True)
                 while True:
                     callback(self)
                     if self.time_index == self.time_steps - 1:
@@ -192,7 +206,14 @@ class MeshSimulation:
                 done = sess.step_index()
                 while done < total:
                     k = min(chunk_steps, total - done)
-                    sess.ctx.run(k)
+                    if self.spec.checkpoints and done + k < total:
+                        # end the chunk at an exact state for the checkpoint
+                        sess.ctx.run(k - 1)
+                        sess.hold(True)
+                        sess.ctx.run(1)
+                        sess.hold(False)
+                    else:
+                        sess.ctx.run(k)
                     done += k
                     if self.spec.checkpoints and done < total:
                         now = datetime.datetime.utcnow()

@@ -124,3 +124,61 @@ def fixed_step_count(spec, n_steps):
         tf = np.nextafter(tf, np.inf)
     spec.time_final = tf
     return spec
+
+
+def cyl_scan_params(n=32, seed=0):
+    """
+    Seeded draws for the batched CylindricalSliceMesh ADI workload: per
+    simulation, in this order, pulse width sigma ~ U[100, 500] as and peak field
+    E0 ~ logU[0.005, 0.1] atomic units (GaussianPulse.from_amplitude with the
+    default carrier, pulses.py:1151-1175). Mesh z in [-100, 100] bohr with 2000
+    points, rho in (0, 50] bohr with 1000 points; time window [-5 sigma, 5 sigma],
+    dt = 1 as; RadialCosineMask over the outer 10 % of rho_bound (45 -> 50 bohr).
+    """
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        pw = float(rng.uniform(100, 500)) * u.asec
+        e0 = float(np.exp(rng.uniform(np.log(0.005), np.log(0.1)))) * u.atomic_electric_field
+        out.append(dict(index=i, pulse_width=pw, amplitude=e0))
+    return out
+
+
+def cyl_scan_spec(p, *, z_points=2000, rho_points=1000, store_data_every=100):
+    from .mesh import CylindricalSliceSpecification
+
+    pw = p["pulse_width"]
+    return CylindricalSliceSpecification(
+        f"cyl_{p['index']}", z_bound=100 * u.bohr_radius, rho_bound=50 * u.bohr_radius, z_points=z_points,
+        rho_points=rho_points,
+        electric_potential=potentials.GaussianPulse.from_amplitude(pulse_width=pw, amplitude=p["amplitude"]),
+        mask=potentials.RadialCosineMask(45 * u.bohr_radius, 50 * u.bohr_radius),
+        time_initial=-5 * pw, time_final=5 * pw, time_step=1 * u.asec, store_data_every=store_data_every)
+
+
+def cyl_scan_compiled(params, **kw):
+    """
+    Compile every simulation of the cylindrical scan. All share one mesh, so the
+    hydrogen 1s start (and test state) is evaluated once and reused.
+    """
+    from .mesh import evolution_methods
+
+    compiled = []
+    g0 = None
+    for p in params:
+        spec = cyl_scan_spec(p, **kw)
+        if g0 is None:
+            sim = spec.to_sim()
+            g0 = sim.mesh.g
+            times = sim.times
+        else:
+            total = spec.time_final - spec.time_initial
+            times = np.linspace(spec.time_initial, spec.time_final, int(total / spec.time_step) + 1)
+        pot = spec.electric_potential
+        compiled.append(engine.compile_cyl(
+            z_bound=spec.z_bound, rho_bound=spec.rho_bound, z_points=spec.z_points, rho_points=spec.rho_points,
+            field=lambda t, pot=pot: np.asarray(pot.get_electric_field_amplitude(t), dtype=np.float64),
+            times=times, g0=g0, basis=[g0], store_every=spec.store_data_every,
+            charge=evolution_methods._coulomb_charge(spec.internal_potential), test_mass=spec.test_mass,
+            test_charge=spec.test_charge, mask=evolution_methods._radial_mask(spec.mask)))
+    return compiled

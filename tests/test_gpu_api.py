@@ -126,3 +126,40 @@ def test_imaginary_time_ground_state_matches_lapack():
     v = v[:, 0] / np.sqrt(orc.delta_z * np.sum(v[:, 0] ** 2))
     overlap = abs(np.vdot(v, g)) * orc.delta_z
     assert abs(overlap - 1) < 1e-10
+
+
+def _cyl_spec(name="cyl", **kw):
+    args = dict(z_bound=20 * u.bohr_radius, rho_bound=10 * u.bohr_radius, z_points=120, rho_points=60,
+                test_states=[states.HydrogenBoundState(1, 0), states.HydrogenBoundState(2, 1)],
+                electric_potential=potentials.SincPulse(pulse_width=60 * u.asec, fluence=3.0 * u.Jcm2, phase=0.2),
+                mask=potentials.RadialCosineMask(14 * u.bohr_radius, 20 * u.bohr_radius),
+                time_initial=-30 * u.asec, time_final=30 * u.asec, time_step=1 * u.asec, store_data_every=5)
+    args.update(kw)
+    return mesh.CylindricalSliceSpecification(name, **args)
+
+
+def test_cyl_spec_run_matches_oracle():
+    spec = _cyl_spec()
+    sim = spec.to_sim()
+    orc = mesh_cpu.CylOracle(
+        z_bound=spec.z_bound, rho_bound=spec.rho_bound, z_points=spec.z_points, rho_points=spec.rho_points,
+        potential=physics.coulomb, field=lambda t: spec.electric_potential.get_electric_field_amplitude(t),
+        mask=lambda r: physics.radial_cosine_mask(r, 14 * u.bohr_radius, 20 * u.bohr_radius))
+    basis = [sim.mesh.get_g_for_state(s) for s in spec.test_states]
+    g, norms, ips = orc.run(sim.mesh.g, sim.times, 5, basis=basis)
+    sim.run()
+    assert rel_l2(sim.mesh.g, g) <= 1e-10
+    assert np.max(np.abs(sim.data.norm - norms)) <= 1e-9
+    dev_ips = np.array([sim.data.inner_products[s] for s in spec.test_states]).T
+    assert np.max(np.abs(dev_ips - ips)) <= 1e-9
+
+
+def test_cyl_callback_mode_matches_batch_mode():
+    batch = _cyl_spec("b").to_sim()
+    batch.run()
+    seen = []
+    cb = _cyl_spec("c").to_sim()
+    cb.run(callback=lambda s: seen.append(s.time_index))
+    assert seen == list(range(cb.time_steps))
+    assert rel_l2(cb.mesh.g, batch.mesh.g) <= 1e-13
+    assert np.max(np.abs(cb.data.norm - batch.data.norm)) <= 1e-13
