@@ -140,9 +140,9 @@ int tdse_line_add(tdse_ctx *ctx, const tdse_line_desc *desc, const double *table
  * (mesh_operators.py:244-298). Device layout: g[rho][p], p < npz =
  * z_chunks * z_rows, position p holding z index (p % z_chunks) * z_rows +
  * p / z_chunks; rho padded to nrpad = rho_chunks * rho_rows.
- * The rho-sweep shape (rho_rows, lanes, rho_chunks) is one of (8, 8, <= 128),
- * (8, 4, <= 128), (4, 8, <= 128), (16, 4, <= 128), with rho_chunks * lanes a
- * multiple of 64.
+ * The rho-sweep shape (rho_rows, lanes, rho_chunks) is one of (4, 4, <= 256),
+ * (8, 4, <= 128), (4, 8, <= 128), (8, 8, <= 128), (16, 4, <= 128), with
+ * rho_chunks * lanes a multiple of 64.
  */
 typedef struct {
     int32_t z_points, rho_points;

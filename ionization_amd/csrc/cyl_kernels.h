@@ -57,11 +57,21 @@ struct CylSim {
     int hold;   // 1: end every step at an exact state (phase 1)
 };
 
+// 1 / sqrt(x): hardware estimate refined by two Newton steps
+// y <- y + y (1 - x y^2) / 2 (relative error well below 1e-15).
+__device__ __forceinline__ double cyl_rsqrt(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-(x * y), y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-(x * y), y, 1.0);
+    y = fma(0.5 * y, e, y);
+    return y;
+}
+
 // Half the Coulomb potential plus half the interaction at (z, rho):
 // internal_hamiltonian adds V / 2, total_hamiltonian adds (-z q E) / 2.
 __device__ __forceinline__ double cyl_half_potential(const CylSim& S, double z, double rho, double E) {
-    const double r = sqrt(z * z + rho * rho);
-    return 0.5 * (S.kc / r) + 0.5 * (-z * S.q * E);
+    return 0.5 * (S.kc * cyl_rsqrt(z * z + rho * rho)) + 0.5 * (-z * S.q * E);
 }
 
 // RadialCosineMask.__call__ (potentials/masks.py:76-89) at r.
@@ -162,8 +172,8 @@ __device__ __forceinline__ double cyl_block_sum(double v, double* red, int t, in
     return s;
 }
 
-template <int M, int LANES, int MODE, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_cyl_rho(CylSim* __restrict__ sims, const int2* __restrict__ groups,
+template <int M, int LANES, int MODE, int MAXT, int WPE>
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 8))) k_cyl_rho(CylSim* __restrict__ sims, const int2* __restrict__ groups,
                                                   int n_groups) {
     extern __shared__ double2 lds[];  // 3 * Cr * LANES double2, then 16 doubles for reductions
     const int gi = blockIdx.x;

@@ -241,7 +241,7 @@ int rebuild_tables(tdse_ctx* ctx) {
     }
     std::vector<int2> cjobs;
     std::map<std::pair<int, int>, std::vector<int2>> czb;             // (rows, chunks)
-    std::map<std::pair<int, int>, std::vector<int2>> crb;             // (rows, chunks * 16 + lanes)
+    std::map<std::pair<int, int>, std::vector<int2>> crb;             // (rows + 64 * lanes, chunks)
     for (int s = 0; s < (int)ctx->cyl.size(); ++s) {
         const CylHost& h = ctx->cyl[s];
         if (!h.live) continue;
@@ -250,7 +250,7 @@ int rebuild_tables(tdse_ctx* ctx) {
         for (int b = 0; b < C.n_basis; ++b) cjobs.push_back(make_int2(s, b));
         auto& vz = czb[{C.Mz, C.Pz}];
         for (int r = 0; r < C.nr; ++r) vz.push_back(make_int2(s, r));
-        auto& vr = crb[{C.Mr, C.Cr * 16 + C.lanes}];
+        auto& vr = crb[{C.Mr + 64 * C.lanes, C.Cr}];
         for (int g = 0; g < C.n_groups; ++g) vr.push_back(make_int2(s, g));
     }
     int rc;
@@ -276,10 +276,9 @@ int rebuild_tables(tdse_ctx* ctx) {
     if ((rc = fill(ctx->line_buckets, lnb))) return rc;
     if ((rc = fill(ctx->cyl_z_buckets, czb))) return rc;
     if ((rc = fill(ctx->cyl_r_buckets, crb))) return rc;
-    for (auto& b : ctx->cyl_r_buckets) {  // key second = chunks * 16 + lanes
-        const int key = b.P;
-        b.P = key / 16;
-        b.lanes = key % 16;
+    for (auto& b : ctx->cyl_r_buckets) {  // key first = rows + 64 * lanes
+        b.lanes = b.M / 64;
+        b.M %= 64;
     }
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     drop_graph(ctx);
@@ -374,25 +373,26 @@ void dispatch_cyl_z(tdse_ctx* ctx, const Bucket& b) {
     }
 }
 
-template <int M, int LANES, int MAXT>
+template <int M, int LANES, int MAXT, int WPE>
 void launch_cyl_rho(tdse_ctx* ctx, const Bucket& b, int mode) {
     const size_t lds = 3 * (size_t)b.P * LANES * sizeof(double2) + 16 * sizeof(double);
     const dim3 grid((unsigned)b.lines.count), block((unsigned)(b.P * LANES));
     if (mode == CYL_PRE)
-        hipLaunchKernelGGL((k_cyl_rho<M, LANES, CYL_PRE, MAXT>), grid, block, lds, ctx->stream, ctx->d_cyl,
+        hipLaunchKernelGGL((k_cyl_rho<M, LANES, CYL_PRE, MAXT, WPE>), grid, block, lds, ctx->stream, ctx->d_cyl,
                            b.lines.ptr, (int)b.lines.count);
     else
-        hipLaunchKernelGGL((k_cyl_rho<M, LANES, CYL_MAIN, MAXT>), grid, block, lds, ctx->stream, ctx->d_cyl,
+        hipLaunchKernelGGL((k_cyl_rho<M, LANES, CYL_MAIN, MAXT, WPE>), grid, block, lds, ctx->stream, ctx->d_cyl,
                            b.lines.ptr, (int)b.lines.count);
 }
 
-// rho-sweep shapes (rows, lines per workgroup, block limit); the layout chooser
-// of the host package picks one of these.
+// rho-sweep shapes (rows, lines per workgroup, block limit, minimum waves per
+// EU); the layout chooser of the host package picks one of these.
 void dispatch_cyl_rho(tdse_ctx* ctx, const Bucket& b, int mode) {
-    if (b.M == 8 && b.lanes == 8) launch_cyl_rho<8, 8, 1024>(ctx, b, mode);
-    else if (b.M == 8 && b.lanes == 4) launch_cyl_rho<8, 4, 512>(ctx, b, mode);
-    else if (b.M == 4 && b.lanes == 8) launch_cyl_rho<4, 8, 1024>(ctx, b, mode);
-    else launch_cyl_rho<16, 4, 512>(ctx, b, mode);
+    if (b.M == 4 && b.lanes == 4) launch_cyl_rho<4, 4, 1024, 4>(ctx, b, mode);
+    else if (b.M == 8 && b.lanes == 4) launch_cyl_rho<8, 4, 512, 4>(ctx, b, mode);
+    else if (b.M == 4 && b.lanes == 8) launch_cyl_rho<4, 8, 1024, 4>(ctx, b, mode);
+    else if (b.M == 8 && b.lanes == 8) launch_cyl_rho<8, 8, 1024, 4>(ctx, b, mode);
+    else launch_cyl_rho<16, 4, 512, 2>(ctx, b, mode);
 }
 
 // Launch one full step of every simulation kind onto the context stream.
@@ -818,8 +818,9 @@ int tdse_cyl_add(tdse_ctx* ctx, const tdse_cyl_desc* desc, const double* zt, con
     if (Mz < 2 || Mz > 16 || (Mz & 1)) return fail(ctx, TDSE_ERR_ARG, "tdse_cyl_add: z_rows must be even and in [2, 16]");
     if ((Mz <= 8 && Pz > 512) || (Mz > 8 && Pz > 256)) return fail(ctx, TDSE_ERR_ARG, "tdse_cyl_add: too many z chunks for z rows");
     if ((int64_t)Pz * Mz < D.z_points) return fail(ctx, TDSE_ERR_ARG, "tdse_cyl_add: z_chunks * z_rows < z_points");
-    const bool shape_ok = (Mr == 8 && lanes == 8 && Cr * 8 <= 1024) || (Mr == 8 && lanes == 4 && Cr * 4 <= 512) ||
-                          (Mr == 4 && lanes == 8 && Cr * 8 <= 1024) || (Mr == 16 && lanes == 4 && Cr * 4 <= 512);
+    const bool shape_ok = (Mr == 4 && lanes == 4 && Cr * 4 <= 1024) || (Mr == 8 && lanes == 4 && Cr * 4 <= 512) ||
+                          (Mr == 4 && lanes == 8 && Cr * 8 <= 1024) || (Mr == 8 && lanes == 8 && Cr * 8 <= 1024) ||
+                          (Mr == 16 && lanes == 4 && Cr * 4 <= 512);
     if (!shape_ok || Cr < 1 || (Cr * lanes) % TDSE_WAVE)
         return fail(ctx, TDSE_ERR_ARG, "tdse_cyl_add: unsupported rho layout (rows, lanes, chunks)");
     if ((int64_t)Cr * Mr < D.rho_points) return fail(ctx, TDSE_ERR_ARG, "tdse_cyl_add: rho_chunks * rho_rows < rho_points");

@@ -291,9 +291,9 @@ def compile_line(*, z_bound, z_points, potential, field: Callable, mask, times, 
     )
 
 
-# rho-sweep shapes the library instantiates: (rows, lines per workgroup), in order of preference
-CYL_RHO_SHAPES = ((8, 4), (4, 8), (16, 4), (8, 8))
-CYL_RHO_MAX_CHUNKS = 128
+# rho-sweep shapes the library instantiates: (rows, lines per workgroup) -> most chunks
+# per line (threads per workgroup / lines), in order of preference
+CYL_RHO_SHAPES = {(4, 4): 256, (8, 4): 128, (4, 8): 128, (16, 4): 128, (8, 8): 128}
 
 
 def choose_cyl_layout(z_points: int, rho_points: int, z_rows: Optional[int] = None,
@@ -313,7 +313,7 @@ def choose_cyl_layout(z_points: int, rho_points: int, z_rows: Optional[int] = No
         step = 64 // lanes
         cr = -(-rho_points // mr)
         cr = -(-cr // step) * step
-        if cr > CYL_RHO_MAX_CHUNKS:
+        if cr > CYL_RHO_SHAPES[(mr, lanes)]:
             continue
         key = (cr * mr - rho_points, pref)
         if best is None or key < best[0]:

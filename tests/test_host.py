@@ -189,3 +189,52 @@ def test_fixed_step_count():
 def test_hamiltonian_derivation_is_rejected():
     with pytest.raises(NotImplementedError):
         mesh.SphericalHarmonicLengthGaugeOperators(kinetic_energy_derivation="hamiltonian")
+
+
+# ------------------------------------------------------------ cylindrical layout
+
+
+@pytest.mark.parametrize("nz, nr", [(2000, 1000), (120, 60), (64, 33), (301, 129), (150, 44)])
+def test_cyl_layout_covers_mesh_and_round_trips(nz, nr):
+    Pz, Mz, Cr, Mr, lanes = engine.choose_cyl_layout(nz, nr)
+    assert Pz % 64 == 0 and Pz * Mz >= nz
+    assert (Mr, lanes) in engine.CYL_RHO_SHAPES and Cr <= engine.CYL_RHO_SHAPES[(Mr, lanes)]
+    assert Cr * Mr >= nr and (Cr * lanes) % 64 == 0 and (Pz * Mz) % lanes == 0
+    rng = np.random.default_rng(nz + nr)
+    g = rng.standard_normal((nz, nr)) + 1j * rng.standard_normal((nz, nr))
+    d = engine.to_cyl_positions(g, Pz, Mz, Cr * Mr)
+    assert d.shape == (Cr * Mr, Pz * Mz)
+    assert np.count_nonzero(d) == nz * nr  # padding is zero
+    assert np.array_equal(engine.from_cyl_positions(d, nz, nr, Pz, Mz), g)
+
+
+def test_cyl_layout_forced_shapes_and_rejects_unknown():
+    for shape in engine.CYL_RHO_SHAPES:
+        _, _, Cr, Mr, lanes = engine.choose_cyl_layout(120, 60, rho_shape=shape)
+        assert (Mr, lanes) == shape
+    with pytest.raises(exceptions.DeviceError):
+        engine.choose_cyl_layout(120, 60, rho_shape=(6, 4))
+    with pytest.raises(exceptions.DeviceError):
+        engine.choose_cyl_layout(120, 5000)
+
+
+def test_cyl_compile_tables_follow_reference_grid():
+    """z / rho grids, the rho coupling c(j) = j / sqrt(j^2 - 1/4) and tau_n (meshes.py:533-544,
+    mesh_operators.py:578-600) in the device tables."""
+    zb, rb = 20 * u.bohr_radius, 10 * u.bohr_radius
+    times = np.linspace(-10 * u.asec, 10 * u.asec, 21)
+    nz, nr = 90, 31
+    g0 = np.ones((nz, nr), np.complex128)
+    c = engine.compile_cyl(z_bound=zb, rho_bound=rb, z_points=nz, rho_points=nr, field=lambda t: 0 * t,
+                           times=times, g0=g0)
+    z = np.linspace(-zb, zb, nz)
+    rho = np.delete(np.linspace(0, rb, nr + 1), 0)
+    drho = rho[1] - rho[0]
+    jz = engine.position_index(nz, c.Pz, c.Mz)
+    assert np.array_equal(c.zt[jz < nz], z[jz[jz < nz]])
+    assert np.array_equal(c.rt[:nr], rho - drho / 2) and not c.rt[nr:].any()
+    rp = -(u.hbar ** 2) / (2 * u.electron_mass_reduced * drho ** 2)
+    j = np.arange(1, nr)
+    assert np.allclose(c.roff[: nr - 1], rp * j / np.sqrt(j ** 2 - 0.25), rtol=1e-15, atol=0)
+    assert not c.roff[nr - 1:].any()
+    assert np.allclose(c.ts, np.diff(times) / (2 * u.hbar), rtol=1e-15, atol=0)
