@@ -132,6 +132,53 @@ def time_cyl(n_sims, seed, steps, warmup):
     return out
 
 
+def time_line(n_batch, cpu_seconds):
+    """
+    LineMesh CN, 8192 points x 4000 steps (soft Coulomb, sine field, one-sided
+    mask): the single simulation as one launch (latency), a batch of n_batch
+    copies in one launch (throughput), and the oracle on one core for a bounded
+    number of steps.
+    """
+    from ionization_amd.engine import DeviceContext, KIND_LINE
+
+    ctx = DeviceContext(pool_bytes=(n_batch + 2) * (2 << 20) + (64 << 20))
+    from ionization_amd import workloads
+
+    spec = workloads.line_cn_spec(z_points=8192, n_steps=4000, store_data_every=100, ctx=ctx)
+    sim = spec.to_sim()
+    comp = spec.evolution_method.compile(sim)
+    slot = ctx.add_line(comp)
+    ms_single = ctx.line_run(comp.n_steps)
+    ctx.remove(KIND_LINE, slot)
+    slots = [ctx.add_line(comp) for _ in range(n_batch)]
+    ms_batch = ctx.line_run(comp.n_steps)
+    for s in slots:
+        ctx.remove(KIND_LINE, s)
+    ctx.close()
+    work = comp.z_points * comp.n_steps
+    out = {"workload": "line_cn: LineMesh 8192 points x 4000 steps, soft Coulomb, sine field, CN (one launch)",
+           "single_ms": ms_single, "single_value": work / (ms_single * 1e-3),
+           "batch": n_batch, "batch_ms": ms_batch, "batch_value": n_batch * work / (ms_batch * 1e-3),
+           "unit": "mesh-point-steps/s"}
+    if cpu_seconds > 0:
+        from oracle import mesh_cpu
+
+        orc = mesh_cpu.LineOracle(
+            z_bound=spec.z_bound, z_points=spec.z_points,
+            potential=lambda z: spec.internal_potential(r=z, z=z, test_charge=spec.test_charge),
+            field=lambda t: spec.electric_potential.get_electric_field_amplitude(t),
+            mask=lambda z: spec.mask(r=z), test_mass=spec.test_mass)
+        g = np.array(sim.mesh.g)
+        steps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds and steps < comp.n_steps:
+            g = orc.evolve(g, sim.times[steps + 1] - sim.times[steps], sim.times[steps + 1])
+            steps += 1
+        out["cpu_port_value"] = comp.z_points * steps / (time.perf_counter() - t0)
+    log(f"line: single {ms_single:.1f} ms ({out['single_value'] / 1e9:.3f} G/s), "
+        f"batch of {n_batch} {ms_batch:.1f} ms ({out['batch_value'] / 1e9:.2f} G/s)")
+    return out
+
+
 def cpu_baseline(params, seconds):
     """The oracle on one core: one simulation of the scan, as many steps as fit in about `seconds`."""
     from ionization_amd import units as u
@@ -291,6 +338,7 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--cyl-sims", type=int, default=32, help="0 skips the cylindrical ADI workload")
     ap.add_argument("--cyl-steps", type=int, default=100)
+    ap.add_argument("--line-batch", type=int, default=256, help="0 skips the LineMesh workload")
     ap.add_argument("--summarize-profiles", metavar="DIR")
     ap.add_argument("--trace-command", default="")
     ap.add_argument("--fetch-command", default="")
@@ -335,6 +383,7 @@ def main():
     step_gbs = rate * BYTES_PER_POINT_STEP / 1e9
 
     cyl = time_cyl(args.cyl_sims, args.seed, args.cyl_steps, 10) if args.cyl_sims > 0 else None
+    line = time_line(args.line_batch, min(args.cpu_seconds, 5.0)) if args.line_batch > 0 else None
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -381,6 +430,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "cyl_adi": cyl,
+        "line_cn": line,
     }
     print(json.dumps(out), flush=True)
     if args.calibrate_copy:  # after the JSON line: a known-size kernel for the counter passes

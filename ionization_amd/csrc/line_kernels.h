@@ -112,14 +112,44 @@ __global__ void __launch_bounds__(MAXT) k_line_steps(LineSim* __restrict__ sims,
         const double f = ld1(S.fs, n);
         double2 xl, xr;
         chunk_neighbours<M>(x, k, P, lds, xl, xr, true);
-        auto coef = [&](int i, double& hm, double& h, double& hp) {
-            h = fma(f, ld1(ZQ, i * P), ld1(H, i * P));
-            hm = ld1(AP, i * P);
-            hp = ld1(AN, i * P);
-        };
-        cn_partition<M, IMAG>(x, k, P, xl, xr, w, coef, lds, true);
+        if constexpr (IMAG) {
+            // w = i s with s real: solve with H~ = s H by the low-register solver,
+            // which re-reads the (L2-resident) tables instead of holding them
+            const double sw = w.y;
+            // Opaque per-step copies of the table pointers: the table values are
+            // invariant across steps, and without this the compiler would hoist
+            // every load out of the step loop and hold 5 M doubles per thread.
+            const double* H1 = H;
+            const double* ZQ1 = ZQ;
+            const double* AP1 = AP;
+            const double* AN1 = AN;
+            asm volatile("" : "+v"(H1), "+v"(ZQ1), "+v"(AP1), "+v"(AN1));
+            auto coef = [&](int i, double& hm, double& h, double& hp) {
+                h = sw * fma(f, ld1(ZQ1, i * P), ld1(H1, i * P));
+                hm = sw * ld1(AP1, i * P);
+                hp = sw * ld1(AN1, i * P);
+            };
+            auto coef2 = [&](int i, double& hm, double& h) {
+                const double* H2 = H;
+                const double* ZQ2 = ZQ;
+                const double* AP2 = AP;
+                asm volatile("" : "+v"(H2), "+v"(ZQ2), "+v"(AP2));
+                h = sw * fma(f, ld1(ZQ2, i * P), ld1(H2, i * P));
+                hm = sw * ld1(AP2, i * P);
+            };
+            cn_implicit_lowreg<M, decltype(coef), decltype(coef2), true>(x, k, P, coef, coef2, lds, xl, xr);
+        } else {
+            auto coef = [&](int i, double& hm, double& h, double& hp) {
+                h = fma(f, ld1(ZQ, i * P), ld1(H, i * P));
+                hm = ld1(AP, i * P);
+                hp = ld1(AN, i * P);
+            };
+            cn_partition<M, IMAG>(x, k, P, xl, xr, w, coef, lds, true);
+        }
+        const double* MK1 = MK;
+        asm volatile("" : "+v"(MK1));
 #pragma unroll
-        for (int i = 0; i < M; ++i) x[i] = cscale(ld1(MK, i * P), x[i]);
+        for (int i = 0; i < M; ++i) x[i] = cscale(ld1(MK1, i * P), x[i]);
         if (S.renormalize) {
             // imaginary-time relaxation: restore unit norm after every step
             double acc = 0.0;
