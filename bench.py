@@ -179,6 +179,32 @@ def time_line(n_batch, cpu_seconds):
     return out
 
 
+def time_serve(n, rate, seed, pool_gb):
+    """
+    The served stream: Poisson arrivals (seeded) of 70 % sh_single-like, 20 %
+    sh_scan draws and 10 % line_cn requests; compile on arrival, admission by
+    device footprint, retirement as soon as the last step has run.
+    """
+    from ionization_amd import serve
+
+    gs = serve._line_ground_state()
+    reqs = serve.served_mix(n, rate, seed, line_ground_state=gs)
+    with serve.Server(pool_bytes=int(pool_gb * (1 << 30)), chunk_steps=25, compile_workers=8) as srv:
+        rep = srv.run(reqs)
+    kinds = {}
+    for r in rep.requests:
+        kinds.setdefault(r.kind, []).append(r.latency)
+    out = {"workload": f"served_stream: {n} requests, Poisson {rate}/s (seed {seed}), 70 % sh_single-like "
+                       "(200 x 1000, 8000 steps), 20 % sh_scan draws, 10 % line_cn (8192 x 4000); "
+                       f"{pool_gb} GiB admission pool",
+           "value": rep.throughput, "unit": "mesh-point-steps/s", "p50_s": rep.p50, "p99_s": rep.p99,
+           "makespan_s": rep.makespan_s, "replays": rep.replays,
+           "by_kind": {k: {"n": len(v), "p50_s": float(np.percentile(v, 50)), "max_s": float(np.max(v))}
+                       for k, v in kinds.items()}}
+    log(f"serve: {n} requests, p50 {rep.p50:.2f} s, p99 {rep.p99:.2f} s, {rep.throughput / 1e9:.2f} G/s")
+    return out
+
+
 def cpu_baseline(params, seconds):
     """The oracle on one core: one simulation of the scan, as many steps as fit in about `seconds`."""
     from ionization_amd import units as u
@@ -339,6 +365,9 @@ def main():
     ap.add_argument("--cyl-sims", type=int, default=32, help="0 skips the cylindrical ADI workload")
     ap.add_argument("--cyl-steps", type=int, default=100)
     ap.add_argument("--line-batch", type=int, default=256, help="0 skips the LineMesh workload")
+    ap.add_argument("--serve-requests", type=int, default=96, help="0 skips the served stream")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-pool-gb", type=float, default=96.0)
     ap.add_argument("--summarize-profiles", metavar="DIR")
     ap.add_argument("--trace-command", default="")
     ap.add_argument("--fetch-command", default="")
@@ -384,6 +413,8 @@ def main():
 
     cyl = time_cyl(args.cyl_sims, args.seed, args.cyl_steps, 10) if args.cyl_sims > 0 else None
     line = time_line(args.line_batch, min(args.cpu_seconds, 5.0)) if args.line_batch > 0 else None
+    served = (time_serve(args.serve_requests, args.serve_rate, args.seed, args.serve_pool_gb)
+              if args.serve_requests > 0 else None)
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -431,6 +462,7 @@ def main():
         "cpu_baseline": cpu,
         "cyl_adi": cyl,
         "line_cn": line,
+        "served_stream": served,
     }
     print(json.dumps(out), flush=True)
     if args.calibrate_copy:  # after the JSON line: a known-size kernel for the counter passes
