@@ -692,11 +692,12 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
     S.active = 1;
     H.n_data = n_data_points(S.N, S.store_every);
     const size_t npad = S.npad;
-    double2 *g, *snap, *oip;
+    double2 *g, *g2, *snap, *oip;
     double *tab, *dfs, *dts, *part, *on, *oln, *bas = nullptr;
     int* bl = nullptr;
     int rc = 0;
     if ((rc = copy_to_device(ctx, H.blocks, reinterpret_cast<const double2*>(g0), (size_t)S.L * npad, &g)) ||
+        (rc = copy_to_device<double2>(ctx, H.blocks, nullptr, (size_t)S.L * npad, &g2)) ||
         (rc = copy_to_device(ctx, H.blocks, tables, SH_NTAB * npad, &tab)) ||
         (rc = copy_to_device(ctx, H.blocks, fs, (size_t)S.N, &dfs)) ||
         (rc = copy_to_device(ctx, H.blocks, ts, (size_t)S.N, &dts)) ||
@@ -712,6 +713,7 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
         return rc;
     }
     S.g = g;
+    S.g2 = g2;
     S.tab = tab;
     S.fs = dfs;
     S.ts = dts;
@@ -1005,7 +1007,12 @@ int tdse_download(tdse_ctx* ctx, int32_t kind, int32_t slot, double* g, double* 
         if (slot < 0 || slot >= (int)ctx->sh.size() || !ctx->sh[slot].live) return fail(ctx, TDSE_ERR_ARG, "bad SH slot");
         const ShHost& H = ctx->sh[slot];
         const ShSim& S = H.dev;
-        if (g) HIP_TRY(ctx, hipMemcpy(g, S.g, (size_t)S.L * S.npad * sizeof(double2), hipMemcpyDeviceToHost));
+        if (g) {
+            int st = 0;
+            HIP_TRY(ctx, hipMemcpy(&st, reinterpret_cast<char*>(ctx->d_sh + slot) + offsetof(ShSim, step), sizeof(int),
+                                   hipMemcpyDeviceToHost));
+            HIP_TRY(ctx, hipMemcpy(g, sh_state(S, st), (size_t)S.L * S.npad * sizeof(double2), hipMemcpyDeviceToHost));
+        }
         if (norms) HIP_TRY(ctx, hipMemcpy(norms, S.out_norm, H.n_data * sizeof(double), hipMemcpyDeviceToHost));
         if (lnorms) HIP_TRY(ctx, hipMemcpy(lnorms, S.out_lnorm, H.n_data * sizeof(double), hipMemcpyDeviceToHost));
         if (ips && S.n_basis > 0)
@@ -1041,7 +1048,7 @@ int tdse_upload_state(tdse_ctx* ctx, int32_t kind, int32_t slot, const double* g
         if (slot < 0 || slot >= (int)ctx->sh.size() || !ctx->sh[slot].live) return fail(ctx, TDSE_ERR_ARG, "bad SH slot");
         const ShSim& S = ctx->sh[slot].dev;
         if (step < 0 || step > S.N + 1) return fail(ctx, TDSE_ERR_ARG, "step out of range");
-        HIP_TRY(ctx, hipMemcpy(S.g, g, (size_t)S.L * S.npad * sizeof(double2), hipMemcpyHostToDevice));
+        HIP_TRY(ctx, hipMemcpy(sh_state(S, step), g, (size_t)S.L * S.npad * sizeof(double2), hipMemcpyHostToDevice));
         const int st[2] = {step, 1};  // the uploaded g is the reference g after `step` steps
         HIP_TRY(ctx, hipMemcpy(reinterpret_cast<char*>(ctx->d_sh + slot) + offsetof(ShSim, step), &st[0], sizeof(int),
                                hipMemcpyHostToDevice));
@@ -1135,7 +1142,13 @@ int tdse_materialize(tdse_ctx* ctx, int32_t kind, int32_t slot) {
     hipLaunchKernelGGL(k_sh_lphase, dim3((nw * TDSE_WAVE + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh,
                        ctx->waves.ptr + H.wave_first, nw, ctx->d_cl, 1);
     HIP_TRY(ctx, hipGetLastError());
+    // the end-only L-phase wrote sh_state(n + 1): swap the buffers so the exact
+    // state is sh_state(n) for the unchanged step counter
+    ShSim& S = ctx->sh[slot].dev;
+    std::swap(S.g, S.g2);
     const int one = 1;
+    HIP_TRY(ctx, hipMemcpyAsync(base + offsetof(ShSim, g), &S.g, sizeof(S.g), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(base + offsetof(ShSim, g2), &S.g2, sizeof(S.g2), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(base + offsetof(ShSim, phase), &one, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return TDSE_OK;

@@ -28,7 +28,8 @@
 
 // Per-simulation device descriptor.
 struct ShSim {
-    double2* g;            // [L][npad]
+    double2* g;            // [L][npad] state before even steps (step counter n even)
+    double2* g2;           // [L][npad] state before odd steps
     const double* tab;     // [SH_NTAB][npad] radial tables in position order
     const double* fs;      // [N] field times tau ratio per step: E(t_{n+1}+dt/2) * tau_n / tau0
     const double* ts;      // [N] tau_n / tau0
@@ -48,6 +49,13 @@ struct ShSim {
 };
 
 enum { SH_T = 0, SH_U = 1, SH_AP = 2, SH_AN = 3, SH_RQ = 4, SH_MASK = 5, SH_NTAB = 6 };
+
+// The wavefunction ping-pongs between two buffers: the state with step
+// counter n lives in sh_state(S, n). The L-phase of a step reads
+// sh_state(S, n) and writes sh_state(S, n + 1) (its waves read input rows
+// outside the l-segment they store, so it must not update in place); the
+// R-phase then works in place on sh_state(S, n + 1).
+__device__ __host__ __forceinline__ double2* sh_state(const ShSim& S, int n) { return (n & 1) ? S.g2 : S.g; }
 
 // Time index n is a data time (sims.py:84-95): first, last and every store_every.
 __device__ __forceinline__ bool sh_is_data_step(int n, int N, int every) {
@@ -89,7 +97,8 @@ __device__ __forceinline__ void sh_rotate(double a, double2& x0, double2& x1) {
 // E rotations of a pair, which share the generator sigma_x and commute with
 // the mask (a function of r only), are applied as one rotation by the sum of
 // their angles. end_only != 0 materialises the reference g after n steps for
-// the listed waves without starting step n and without observing.
+// the listed waves without starting step n and without observing; it too
+// writes sh_state(S, n + 1) (the host then swaps the two buffers).
 struct ShWave {
     int sim, p0, lo, hi, part;  // part: index of this wave's norm partial within the simulation
 };
@@ -108,7 +117,7 @@ __global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, con
     const bool do_end = n > 0 && S.phase == 0;
     const bool do_start = n < N && !end_only;
     const bool observe = !end_only && sh_is_data_step(n, N, S.store_every);
-    if (!do_end && !do_start && !observe) return;
+    // Every wave writes its rows to the other buffer, even with nothing to apply (a copy).
     const bool fuse_e = do_end && do_start && !observe;
 
     const int L = S.L, P = S.P, m = S.m, npad = S.npad;
@@ -122,10 +131,11 @@ __global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, con
     const double f_end = do_end ? ld1(S.fs, n - 1) * rq : 0.0;
     const double f_start = do_start ? ld1(S.fs, n) * rq : 0.0;
     const double f_sum = f_end + f_start;
-    double2* __restrict__ g = S.g + p;
-    auto load_row = [g, live, L, npad](int l) -> double2 {
+    const double2* __restrict__ gin = sh_state(S, n) + p;
+    double2* __restrict__ g = sh_state(S, n + 1) + p;
+    auto load_row = [gin, live, L, npad](int l) -> double2 {
         double2 v = make_double2(0.0, 0.0);
-        if (live && l >= 0 && l < L) v = ld2(g, (size_t)l * npad);
+        if (live && l >= 0 && l < L) v = ld2(gin, (size_t)l * npad);
         return v;
     };
     // coupling of the pair starting at virtual row v (0 when the pair does not exist:
@@ -309,7 +319,7 @@ k_sh_rphase(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_line
     const double sn = ld1(S.ts, n);
     const double ll1 = (double)l * (double)(l + 1);
     const double c0 = (l == 0 && k == 0) ? S.corr0 : 0.0;
-    double2* __restrict__ gl = S.g + (size_t)l * npad + k;
+    double2* __restrict__ gl = sh_state(S, n + 1) + (size_t)l * npad + k;
     const double* __restrict__ T = S.tab + SH_T * npad + k;
     const double* __restrict__ U = S.tab + SH_U * npad + k;
     const double* __restrict__ AP = S.tab + SH_AP * npad + k;

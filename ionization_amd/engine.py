@@ -134,10 +134,10 @@ class CompiledSH:
 
     @property
     def footprint_bytes(self):
-        """Device bytes this simulation holds (g, tables, snapshots, basis, data)."""
+        """Device bytes this simulation holds (two g buffers, tables, snapshots, basis, data)."""
         nd = data_count(self.n_steps, self.store_every)
         return int(
-            16 * self.l_bound * self.npad
+            2 * 16 * self.l_bound * self.npad
             + 8 * SH_NTAB * self.npad
             + 16 * self.n_steps
             + 16 * self.lb * self.npad

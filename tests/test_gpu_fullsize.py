@@ -1,0 +1,137 @@
+"""
+The benchmark workloads at their full sizes, checked through properties that
+do not depend on running the CPU oracle at that size:
+  * unitarity: without a mask the norm stays at its initial value (the step
+    is a product of unitary factors, mesh_operators.py / evolution_methods.py);
+  * stationarity: without a field the numeric eigenstate only acquires a phase
+    (tests/mesh/test_sims.py:30-82 in the reference);
+  * absorption: with the mask the norm never grows;
+  * batch independence: a simulation gives the same bits alone and in the batch.
+"""
+
+import numpy as np
+import pytest
+
+from ionization_amd import engine, potentials, workloads
+from ionization_amd import units as u
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_sh(ctx, comp):
+    s = ctx.add_sh(comp)
+    ctx.run(comp.n_steps + 1)
+    r = ctx.download_sh(s)
+    ctx.remove(engine.KIND_SH, s)
+    return r
+
+
+def test_sh_200x1000_8000_steps_unitary_with_field(device_ctx):
+    spec = workloads.sh_single_spec("u", store_data_every=500, numeric_eigenstate_max_energy=0.0)
+    spec.mask = potentials.NoMask()
+    sim = spec.to_sim()
+    comp = spec.evolution_method.compile(sim)
+    assert comp.n_steps == 8000 and comp.r_points == 1000 and comp.l_bound == 200
+    r = _run_sh(device_ctx, comp)
+    print(f"norm drift {np.max(np.abs(r.norms - r.norms[0])):.2e}", flush=True)
+    assert np.max(np.abs(r.norms - r.norms[0])) <= 1e-9
+    assert abs(np.sum(np.abs(r.g) ** 2) * comp.delta_r - r.norms[-1]) <= 1e-12
+    assert np.abs(r.ips[-1, 0]) < 1 - 1e-8  # the pulse moved population out of 1s
+
+
+def test_sh_200x1000_8000_steps_field_free_eigenstate_is_stationary(device_ctx):
+    spec = workloads.sh_single_spec("s", store_data_every=1000, numeric_eigenstate_max_energy=0.0)
+    spec.electric_potential = potentials.NoElectricPotential()
+    spec.mask = potentials.NoMask()
+    sim = spec.to_sim()
+    r = _run_sh(device_ctx, spec.evolution_method.compile(sim))
+    overlap = np.abs(r.ips[:, 0]) ** 2  # initial state is the first (1s) test state
+    print(f"1s overlap deviation {np.max(np.abs(overlap - 1)):.2e}", flush=True)
+    assert np.max(np.abs(overlap - 1)) <= 1e-9
+    assert np.max(np.abs(r.norms - 1)) <= 1e-9
+
+
+def test_line_8192x4000_unitary_without_mask_and_absorbing_with_it(device_ctx):
+    gs = None
+    for masked in (False, True):
+        spec = workloads.line_cn_spec(z_points=8192, n_steps=4000, store_data_every=100, ground_state=gs,
+                                      ctx=device_ctx)
+        if not masked:
+            spec.mask = potentials.NoMask()
+        sim = spec.to_sim()
+        gs = sim.mesh.g
+        comp = spec.evolution_method.compile(sim)
+        s = device_ctx.add_line(comp)
+        device_ctx.line_run(comp.n_steps)
+        g, norms, _ = device_ctx.download_line(s)
+        device_ctx.remove(engine.KIND_LINE, s)
+        if masked:
+            assert np.all(np.diff(norms) <= 1e-12)
+        else:
+            print(f"line norm drift {np.max(np.abs(norms - norms[0])):.2e}", flush=True)
+            assert np.max(np.abs(norms - norms[0])) <= 1e-9
+
+
+def test_cyl_2000x1000_field_free_invariant(device_ctx):
+    """
+    One ADI step is (1 + i tau H_rho)^-1 C_z (1 - i tau H_rho) with C_z the z Cayley factor, so without a
+    field and a mask N steps are S^-1 (C_z C_rho)^N S with S = 1 + i tau H_rho: the norm itself is not
+    conserved (it moves once, by O(tau^2 [H_rho, H_z])), but |S g|^2 is, exactly.
+    """
+    p = workloads.cyl_scan_params(1, 0)[0]
+    spec = workloads.cyl_scan_spec(p, store_data_every=50)
+    spec.mask = potentials.NoMask()
+    spec.electric_potential = potentials.NoElectricPotential()
+    spec.time_final = spec.time_initial + 300 * spec.time_step
+    sim = spec.to_sim()
+    comp = spec.evolution_method.compile(sim)
+    assert comp.z_points == 2000 and comp.rho_points == 1000 and comp.n_steps == 300
+    g0 = sim.mesh.g
+    s = device_ctx.add_cyl(comp)
+    device_ctx.run(comp.n_steps)
+    g, norms, _ = device_ctx.download_cyl(s)
+    device_ctx.remove(engine.KIND_CYL, s)
+
+    nr = comp.rho_points
+    z = np.linspace(-spec.z_bound, spec.z_bound, comp.z_points)[:, None]
+    rho = comp.rt[None, :nr]
+    diag = comp.rho_prefactor * -2.0 + 0.5 * (comp.coulomb / np.sqrt(z * z + rho * rho))
+    off = comp.roff[: nr - 1]
+    tau = comp.ts[0]
+
+    def invariant(v):
+        hv = diag * v
+        hv[:, :-1] += off * v[:, 1:]
+        hv[:, 1:] += off * v[:, :-1]
+        return np.sum(np.abs(v + 1j * tau * hv) ** 2) * comp.delta
+
+    i0, i1 = invariant(g0), invariant(g)
+    print(f"cyl |S g|^2 drift {abs(i1 - i0) / i0:.2e}, norm moved by {norms[-1] - norms[0]:.2e}", flush=True)
+    assert abs(i1 - i0) <= 1e-10 * i0
+    assert np.max(np.abs(norms - norms[0])) <= 1e-5  # the similarity transform moves the norm a little
+
+
+def test_scan_batch_absorbs_and_is_batch_independent(device_ctx):
+    params = workloads.sh_scan_params(48, 0)
+    comps = []
+    for p in params:
+        sim = workloads.sh_scan_spec(p, store_data_every=20).to_sim()
+        c = sim.spec.evolution_method.compile(sim)
+        comps.append(c)
+    steps = 120  # replays: data times 0, 20, ..., 100 have been observed
+    slots = [device_ctx.add_sh(c) for c in comps]
+    device_ctx.run(steps)
+    for c, s in zip(comps, slots):
+        r = device_ctx.download_sh(s, with_g=False)
+        n = r.norms[: steps // 20]
+        assert np.all(np.diff(n) <= 1e-12)
+        assert np.all(n <= 1 + 1e-12)
+    # the first simulation again, alone, for the same number of replays
+    first = device_ctx.download_sh(slots[0]).g
+    for s in slots:
+        device_ctx.remove(engine.KIND_SH, s)
+    s = device_ctx.add_sh(comps[0])
+    device_ctx.run(steps)
+    alone = device_ctx.download_sh(s).g
+    device_ctx.remove(engine.KIND_SH, s)
+    assert np.array_equal(first, alone)
