@@ -287,7 +287,8 @@ def summarize_profiles(d, command_trace, command_fetch, command_write):
         f = _kernel_family(name)
         c, t = fam.get(f, (0, 0.0))
         fam[f] = (c + cnt, t + tot)
-    lines = ["# rocprofv3 kernel trace: 256-simulation SphericalHarmonicMesh scan (bench.py)", "",
+    lines = ["# rocprofv3 kernel trace of bench.py: the 256-simulation SphericalHarmonicMesh scan and the",
+             "# 32-simulation CylindricalSliceMesh workload", "",
              "command:", "    " + command_trace, "",
              f"{'kernel':<70} {'calls':>7} {'total ms':>10} {'avg us':>9} {'min us':>9} {'max us':>9}"]
     for name, cnt, tot, avg, mn, mx in rows:

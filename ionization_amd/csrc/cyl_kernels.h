@@ -89,11 +89,11 @@ __device__ __forceinline__ double cyl_mask(const CylSim& S, double z, double rho
 // One workgroup of Pz threads per (simulation, rho-line); Mz rows per thread.
 // (1 + i tau H_z) y = (1 - i tau H_z) x with H_z tridiagonal: diagonal
 // -2 zp + V/2 - z q E / 2, off-diagonals zp inside the line
-// (mesh_operators.py:561-575). EXPLICIT as for k_sh_rphase.
-template <int M, int MAXT, int WPE, bool EXPLICIT>
+// (mesh_operators.py:561-575). FORM as for k_sh_rphase.
+template <int M, int MAXT, int WPE, int FORM>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_cyl_z(CylSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines) {
-    extern __shared__ double2 lds[];  // 3 * Pz
+    extern __shared__ double2 lds[];  // 3 * Pz, then M * Pz for CN_CAYLEY_LDS
     const int line = blockIdx.x;
     if (line >= n_lines) return;
     const int2 lt = lines[line];
@@ -130,20 +130,7 @@ k_cyl_z(CylSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines) 
         h = live ? fma(tau, cyl_half_potential(S, z, rho, E), tdiag) : 0.0;
         hm = (live && j >= 1) ? tzp : 0.0;
     };
-    if constexpr (EXPLICIT) {
-        double2 xl, xr;
-        chunk_neighbours<M>(x, k, P, lds, xl, xr, false);
-        cn_implicit_lowreg<M, decltype(coef), decltype(coef2), true>(x, k, P, coef, coef2, lds, xl, xr);
-#pragma unroll
-        for (int i = 0; i < M; ++i) st2(gl, (size_t)i * P, x[i]);
-    } else {
-        cn_implicit_lowreg<M>(x, k, P, coef, coef2, lds);
-#pragma unroll
-        for (int i = 0; i < M; ++i) {
-            const double2 x0 = ld2(gl, (size_t)i * P);
-            st2(gl, (size_t)i * P, cmk(fma(2.0, x[i].x, -x0.x), fma(2.0, x[i].y, -x0.y)));
-        }
-    }
+    cn_line_finish<M, FORM>(x, gl, k, P, coef, coef2, lds);
 }
 
 // ---------------------------------------------------------------- rho-sweep

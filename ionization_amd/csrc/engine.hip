@@ -24,6 +24,7 @@ constexpr int kAbiVersion = 1;
 constexpr size_t kAlign = 256;
 constexpr int kMaxL = 4096;          // largest l_bound accepted (c_l table size)
 constexpr int kSlotCapacity = 8192;  // simulations of each kind per context
+constexpr int kSideStreams = 3;      // extra streams for concurrent R-phase buckets (default priority)
 
 thread_local std::string g_lib_error;
 
@@ -163,8 +164,13 @@ struct tdse_ctx {
     DevVec<ShWave> waves;
     DevVec<int2> jobs;
     std::vector<Bucket> sh_buckets;
+    std::vector<int> sh_order;  // sh_buckets by decreasing work
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
     std::vector<Bucket> line_buckets;
     bool dirty = true;
+    bool cyl_pending = false;    // a cylindrical simulation was added or uploaded: it starts from an exact state
+    bool graph_cyl_pre = false;  // the captured step includes the cylindrical pre-sweep
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -276,6 +282,14 @@ int rebuild_tables(tdse_ctx* ctx) {
     if ((rc = fill(ctx->line_buckets, lnb))) return rc;
     if ((rc = fill(ctx->cyl_z_buckets, czb))) return rc;
     if ((rc = fill(ctx->cyl_r_buckets, crb))) return rc;
+    {
+        std::vector<std::pair<double, int>> w;
+        for (size_t i = 0; i < ctx->sh_buckets.size(); ++i)
+            w.push_back({-(double)ctx->sh_buckets[i].lines.count * ctx->sh_buckets[i].P * ctx->sh_buckets[i].M, (int)i});
+        std::sort(w.begin(), w.end());
+        ctx->sh_order.clear();
+        for (auto& pr : w) ctx->sh_order.push_back(pr.second);
+    }
     for (auto& b : ctx->cyl_r_buckets) {  // key first = rows + 64 * lanes
         b.lanes = b.M / 64;
         b.M %= 64;
@@ -305,25 +319,45 @@ void prof_end(tdse_ctx* ctx) {
     (void)hipEventRecord(next_event(ctx), ctx->stream);
 }
 
-// Launch shapes per rows-per-thread M (block size, minimum waves per EU,
-// explicit or Cayley form), chosen from kernel-trace timings of the scan.
-template <int M, int MAXT, int WPE>
-void launch_rphase(tdse_ctx* ctx, const Bucket& b) {
-    const size_t lds = 3 * (size_t)b.P * sizeof(double2);
-    hipLaunchKernelGGL((k_sh_rphase<M, MAXT, WPE, (M >= 10)>), dim3((unsigned)b.lines.count), dim3(b.P), lds, ctx->stream, ctx->d_sh,
-                       b.lines.ptr, (int)b.lines.count);
+// Form of a CN line kernel with M rows per thread and P threads: the
+// explicit right-hand side for M >= 10; for M <= 8 the Cayley form, keeping
+// the original rows in LDS when that fits the 64 KiB a workgroup gets by
+// default, else re-reading them. Chosen from kernel-trace timings.
+int cn_form(int M, int P) {
+    if (M >= 10) return CN_EXPLICIT;
+    return (3 + M) * (size_t)P * sizeof(double2) <= 65536 ? CN_CAYLEY_LDS : CN_CAYLEY_MEM;
+}
+size_t cn_lds_bytes(int M, int P, int form) {
+    return (size_t)(form == CN_CAYLEY_LDS ? 3 + M : 3) * P * sizeof(double2);
 }
 
-void dispatch_rphase(tdse_ctx* ctx, const Bucket& b) {
+// Launch shapes per rows-per-thread M (block size, minimum waves per EU).
+template <int M, int MAXT, int WPE>
+void launch_rphase(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
+    const int form = cn_form(M, b.P);
+    const size_t lds = cn_lds_bytes(M, b.P, form);
+    const dim3 grid((unsigned)b.lines.count), block(b.P);
+    if (form == CN_EXPLICIT)
+        hipLaunchKernelGGL((k_sh_rphase<M, MAXT, WPE, (M >= 10 ? CN_EXPLICIT : CN_CAYLEY_MEM)>), grid, block, lds,
+                           st, ctx->d_sh, b.lines.ptr, (int)b.lines.count);
+    else if (form == CN_CAYLEY_LDS)
+        hipLaunchKernelGGL((k_sh_rphase<M, MAXT, WPE, (M >= 10 ? CN_EXPLICIT : CN_CAYLEY_LDS)>), grid, block, lds,
+                           st, ctx->d_sh, b.lines.ptr, (int)b.lines.count);
+    else
+        hipLaunchKernelGGL((k_sh_rphase<M, MAXT, WPE, (M >= 10 ? CN_EXPLICIT : CN_CAYLEY_MEM)>), grid, block, lds,
+                           st, ctx->d_sh, b.lines.ptr, (int)b.lines.count);
+}
+
+void dispatch_rphase(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
     switch (b.M) {
-        case 2: launch_rphase<2, 512, 4>(ctx, b); break;
-        case 4: launch_rphase<4, 512, 4>(ctx, b); break;
-        case 6: launch_rphase<6, 512, 4>(ctx, b); break;
-        case 8: launch_rphase<8, 512, 4>(ctx, b); break;
-        case 10: launch_rphase<10, 256, 3>(ctx, b); break;
-        case 12: launch_rphase<12, 256, 2>(ctx, b); break;
-        case 14: launch_rphase<14, 256, 2>(ctx, b); break;
-        default: launch_rphase<16, 256, 1>(ctx, b); break;
+        case 2: launch_rphase<2, 512, 4>(ctx, b, st); break;
+        case 4: launch_rphase<4, 512, 4>(ctx, b, st); break;
+        case 6: launch_rphase<6, 512, 4>(ctx, b, st); break;
+        case 8: launch_rphase<8, 512, 4>(ctx, b, st); break;
+        case 10: launch_rphase<10, 256, 3>(ctx, b, st); break;
+        case 12: launch_rphase<12, 256, 2>(ctx, b, st); break;
+        case 14: launch_rphase<14, 256, 2>(ctx, b, st); break;
+        default: launch_rphase<16, 256, 1>(ctx, b, st); break;
     }
 }
 
@@ -355,9 +389,18 @@ void dispatch_line(tdse_ctx* ctx, const Bucket& b, int n_steps) {
 
 template <int M, int MAXT, int WPE>
 void launch_cyl_z(tdse_ctx* ctx, const Bucket& b) {
-    const size_t lds = 3 * (size_t)b.P * sizeof(double2);
-    hipLaunchKernelGGL((k_cyl_z<M, MAXT, WPE, (M >= 10)>), dim3((unsigned)b.lines.count), dim3(b.P), lds, ctx->stream,
-                       ctx->d_cyl, b.lines.ptr, (int)b.lines.count);
+    const int form = cn_form(M, b.P);
+    const size_t lds = cn_lds_bytes(M, b.P, form);
+    const dim3 grid((unsigned)b.lines.count), block(b.P);
+    if (form == CN_EXPLICIT)
+        hipLaunchKernelGGL((k_cyl_z<M, MAXT, WPE, (M >= 10 ? CN_EXPLICIT : CN_CAYLEY_MEM)>), grid, block, lds,
+                           ctx->stream, ctx->d_cyl, b.lines.ptr, (int)b.lines.count);
+    else if (form == CN_CAYLEY_LDS)
+        hipLaunchKernelGGL((k_cyl_z<M, MAXT, WPE, (M >= 10 ? CN_EXPLICIT : CN_CAYLEY_LDS)>), grid, block, lds,
+                           ctx->stream, ctx->d_cyl, b.lines.ptr, (int)b.lines.count);
+    else
+        hipLaunchKernelGGL((k_cyl_z<M, MAXT, WPE, (M >= 10 ? CN_EXPLICIT : CN_CAYLEY_MEM)>), grid, block, lds,
+                           ctx->stream, ctx->d_cyl, b.lines.ptr, (int)b.lines.count);
 }
 
 void dispatch_cyl_z(tdse_ctx* ctx, const Bucket& b) {
@@ -396,7 +439,9 @@ void dispatch_cyl_rho(tdse_ctx* ctx, const Bucket& b, int mode) {
 }
 
 // Launch one full step of every simulation kind onto the context stream.
-int launch_step(tdse_ctx* ctx) {
+// cyl_pre: include the cylindrical pre-sweep (needed while any cylindrical
+// simulation may start a step from an exact state).
+int launch_step(tdse_ctx* ctx, bool cyl_pre) {
     const int nw = (int)ctx->waves.count;
     if (nw > 0) {
         prof_begin(ctx, 0);
@@ -405,10 +450,26 @@ int launch_step(tdse_ctx* ctx) {
         prof_end(ctx);
         const int nj = (int)ctx->jobs.count;
         hipLaunchKernelGGL(k_sh_observe, dim3(nj), dim3(256), 0, ctx->stream, ctx->d_sh, ctx->jobs.ptr, nj);
-        for (auto& b : ctx->sh_buckets) {
-            prof_begin(ctx, 1);
-            dispatch_rphase(ctx, b);
-            prof_end(ctx);
+        if (ctx->profiling || ctx->sh_buckets.size() < 2) {
+            for (auto& b : ctx->sh_buckets) {
+                prof_begin(ctx, 1);
+                dispatch_rphase(ctx, b, ctx->stream);
+                prof_end(ctx);
+            }
+        } else {
+            // independent buckets run concurrently (largest first, round robin
+            // over the context stream and the side streams) so that small
+            // buckets fill the tails of large ones; fork and join by events
+            HIP_TRY(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
+            for (int j = 0; j < kSideStreams; ++j) HIP_TRY(ctx, hipStreamWaitEvent(ctx->side[j], ctx->ev_fork, 0));
+            for (size_t i = 0; i < ctx->sh_order.size(); ++i) {
+                const int j = (int)(i % (kSideStreams + 1));
+                dispatch_rphase(ctx, ctx->sh_buckets[ctx->sh_order[i]], j == 0 ? ctx->stream : ctx->side[j - 1]);
+            }
+            for (int j = 0; j < kSideStreams; ++j) {
+                HIP_TRY(ctx, hipEventRecord(ctx->ev_join[j], ctx->side[j]));
+                HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join[j], 0));
+            }
         }
         const int ns = (int)ctx->sh.size();
         hipLaunchKernelGGL(k_sh_advance, dim3((ns + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh, ns);
@@ -416,8 +477,11 @@ int launch_step(tdse_ctx* ctx) {
     for (auto& b : ctx->line_buckets) dispatch_line(ctx, b, 1);
     if (!ctx->cyl_r_buckets.empty()) {
         const int nj = (int)ctx->cyl_jobs.count;
-        for (auto& b : ctx->cyl_r_buckets) dispatch_cyl_rho(ctx, b, CYL_PRE);
-        hipLaunchKernelGGL(k_cyl_observe, dim3(nj), dim3(256), 0, ctx->stream, ctx->d_cyl, ctx->cyl_jobs.ptr, nj, 0);
+        if (cyl_pre) {
+            for (auto& b : ctx->cyl_r_buckets) dispatch_cyl_rho(ctx, b, CYL_PRE);
+            hipLaunchKernelGGL(k_cyl_observe, dim3(nj), dim3(256), 0, ctx->stream, ctx->d_cyl, ctx->cyl_jobs.ptr, nj,
+                               0);
+        }
         for (auto& b : ctx->cyl_z_buckets) {
             prof_begin(ctx, 2);
             dispatch_cyl_z(ctx, b);
@@ -436,10 +500,12 @@ int launch_step(tdse_ctx* ctx) {
     return TDSE_OK;
 }
 
-int ensure_graph(tdse_ctx* ctx) {
-    if (ctx->exec) return TDSE_OK;
+int ensure_graph(tdse_ctx* ctx, bool cyl_pre) {
+    if (ctx->exec && ctx->graph_cyl_pre == cyl_pre) return TDSE_OK;
+    drop_graph(ctx);
+    ctx->graph_cyl_pre = cyl_pre;
     HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    int rc = launch_step(ctx);
+    int rc = launch_step(ctx, cyl_pre);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (rc) {
@@ -515,7 +581,11 @@ int tdse_ctx_create(int device, size_t pool_bytes, tdse_ctx** out) {
               hipMemset(ctx->d_sh, 0, kSlotCapacity * sizeof(ShSim)) == hipSuccess &&
               hipMemset(ctx->d_line, 0, kSlotCapacity * sizeof(LineSim)) == hipSuccess &&
               hipMemset(ctx->d_cyl, 0, kSlotCapacity * sizeof(CylSim)) == hipSuccess &&
-              hipEventCreate(&ctx->ev_start) == hipSuccess && hipEventCreate(&ctx->ev_stop) == hipSuccess;
+              hipEventCreate(&ctx->ev_start) == hipSuccess && hipEventCreate(&ctx->ev_stop) == hipSuccess &&
+              hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) == hipSuccess;
+    for (int j = 0; ok && j < kSideStreams; ++j)
+        ok = hipStreamCreateWithFlags(&ctx->side[j], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&ctx->ev_join[j], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         tdse_ctx_destroy(ctx);
         return fail(nullptr, TDSE_ERR_HIP, "context allocation failed");
@@ -547,6 +617,11 @@ int tdse_ctx_destroy(tdse_ctx* ctx) {
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    for (int j = 0; j < kSideStreams; ++j) {
+        if (ctx->ev_join[j]) (void)hipEventDestroy(ctx->ev_join[j]);
+        if (ctx->side[j]) (void)hipStreamDestroy(ctx->side[j]);
+    }
     ctx->arena.release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -905,6 +980,7 @@ int tdse_cyl_add(tdse_ctx* ctx, const tdse_cyl_desc* desc, const double* zt, con
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // host buffers may be released after return
     H.live = true;
     ctx->dirty = true;
+    ctx->cyl_pending = true;
     *slot = s;
     return TDSE_OK;
 }
@@ -929,13 +1005,24 @@ int tdse_run(tdse_ctx* ctx, int32_t n_steps, int32_t use_graph) {
     int rc;
     if (ctx->dirty && (rc = rebuild_tables(ctx))) return rc;
     const bool graph = use_graph && !ctx->profiling;
-    if (graph && (rc = ensure_graph(ctx))) return rc;
+    bool hold_any = false;
+    for (const auto& h : ctx->cyl)
+        if (h.live && h.dev.hold) hold_any = true;
     ctx->ev_used = 0;
     ctx->ev_pairs.clear();
     HIP_TRY(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
     for (int i = 0; i < n_steps; ++i) {
-        if (graph) HIP_TRY(ctx, hipGraphLaunch(ctx->exec, ctx->stream));
-        else if ((rc = launch_step(ctx))) return rc;
+        // the pre-sweep is needed for simulations held at exact states, and for
+        // one step after a cylindrical simulation was added or uploaded
+        if (ctx->cyl_pending && !hold_any) {
+            if ((rc = launch_step(ctx, true))) return rc;
+        } else if (graph) {
+            if ((rc = ensure_graph(ctx, hold_any))) return rc;
+            HIP_TRY(ctx, hipGraphLaunch(ctx->exec, ctx->stream));
+        } else if ((rc = launch_step(ctx, hold_any))) {
+            return rc;
+        }
+        ctx->cyl_pending = false;
     }
     HIP_TRY(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
     HIP_TRY(ctx, hipEventSynchronize(ctx->ev_stop));
@@ -1071,6 +1158,7 @@ int tdse_upload_state(tdse_ctx* ctx, int32_t kind, int32_t slot, const double* g
         char* base = reinterpret_cast<char*>(ctx->d_cyl + slot);
         HIP_TRY(ctx, hipMemcpy(base + offsetof(CylSim, step), &st[0], sizeof(int), hipMemcpyHostToDevice));
         HIP_TRY(ctx, hipMemcpy(base + offsetof(CylSim, phase), &st[1], sizeof(int), hipMemcpyHostToDevice));
+        ctx->cyl_pending = true;
     } else {
         return fail(ctx, TDSE_ERR_ARG, "unknown simulation kind");
     }

@@ -289,6 +289,34 @@ __global__ void __launch_bounds__(256) k_sh_observe(ShSim* __restrict__ sims, co
 }
 
 // ------------------------------------------------------------------ R-phase
+
+enum { CN_CAYLEY_MEM = 0, CN_CAYLEY_LDS = 1, CN_EXPLICIT = 2 };
+
+// Shared tail of the CN line kernels: x holds the thread's M rows of the line
+// at gl (stride P); solves and stores y = (1 + i H~)^-1 (1 - i H~) x.
+template <int M, int FORM, class Coef, class Coef2>
+__device__ __forceinline__ void cn_line_finish(double2 (&x)[M], double2* __restrict__ gl, const int k, const int P,
+                                               Coef coef, Coef2 coef2, double2* __restrict__ lds) {
+    if constexpr (FORM == CN_EXPLICIT) {
+        double2 xl, xr;
+        chunk_neighbours<M>(x, k, P, lds, xl, xr, false);
+        cn_implicit_lowreg<M, Coef, Coef2, true>(x, k, P, coef, coef2, lds, xl, xr);
+#pragma unroll
+        for (int i = 0; i < M; ++i) st2(gl, (size_t)i * P, x[i]);
+    } else {
+        double2* keep = lds + 3 * P;  // CN_CAYLEY_LDS: this thread's own rows only
+        if constexpr (FORM == CN_CAYLEY_LDS) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) keep[i * P + k] = x[i];
+        }
+        cn_implicit_lowreg<M>(x, k, P, coef, coef2, lds);
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const double2 x0 = (FORM == CN_CAYLEY_LDS) ? keep[i * P + k] : ld2(gl, (size_t)i * P);
+            st2(gl, (size_t)i * P, cmk(fma(2.0, x[i].x, -x0.x), fma(2.0, x[i].y, -x0.y)));
+        }
+    }
+}
 //
 // (1 + i H~) y = (1 - i H~) x along r for one l-line, H~ = tau_n H0 real
 // symmetric tridiagonal: diagonal s_n (T_j + l(l+1) U_j [+ corr0 at l = 0,
@@ -301,12 +329,15 @@ __global__ void __launch_bounds__(256) k_sh_observe(ShSim* __restrict__ sims, co
 //  * EXPLICIT: the right-hand side (1 - i H~) x is formed inside the solver's
 //    first pass from x and the neighbouring chunks' edge rows; every line is
 //    read once and written once (faster for M >= 10);
-//  * Cayley: z = (1 + i H~)^-1 x, y = 2 z - x, with x re-read at the end
-//    (fewer live registers; faster for M <= 8).
-template <int M, int MAXT, int WPE, bool EXPLICIT>
+//  * Cayley: z = (1 + i H~)^-1 x, y = 2 z - x (fewer live registers; faster
+//    for M <= 8), with the original x kept in LDS (CN_CAYLEY_LDS: M P more
+//    double2 of LDS; each thread reads back only what it wrote) or re-read
+//    from memory at the end (CN_CAYLEY_MEM).
+
+template <int M, int MAXT, int WPE, int FORM>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_sh_rphase(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines) {
-    extern __shared__ double2 lds[];  // 3 * P
+    extern __shared__ double2 lds[];  // 3 * P, then M * P for CN_CAYLEY_LDS
     const int line = blockIdx.x;
     if (line >= n_lines) return;
     const int2 lt = lines[line];
@@ -344,20 +375,7 @@ k_sh_rphase(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_line
         h = sn * (fma(ll1, ld1(U2, i * P), ld1(T2, i * P)) + (i == 0 ? c0 : 0.0));
         hm = sn * ld1(AP2, i * P);
     };
-    if constexpr (EXPLICIT) {
-        double2 xl, xr;
-        chunk_neighbours<M>(x, k, P, lds, xl, xr, false);
-        cn_implicit_lowreg<M, decltype(coef), decltype(coef2), true>(x, k, P, coef, coef2, lds, xl, xr);
-#pragma unroll
-        for (int i = 0; i < M; ++i) st2(gl, (size_t)i * P, x[i]);
-    } else {
-        cn_implicit_lowreg<M>(x, k, P, coef, coef2, lds);
-#pragma unroll
-        for (int i = 0; i < M; ++i) {
-            const double2 x0 = ld2(gl, (size_t)i * P);
-            st2(gl, (size_t)i * P, cmk(fma(2.0, x[i].x, -x0.x), fma(2.0, x[i].y, -x0.y)));
-        }
-    }
+    cn_line_finish<M, FORM>(x, gl, k, P, coef, coef2, lds);
 }
 
 // Advance every active simulation's step counter after a full graph step.
