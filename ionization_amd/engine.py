@@ -39,12 +39,14 @@ LINE_NTAB = 5
 def choose_layout(n: int, kind: str = "sh", rows: Optional[int] = None) -> Tuple[int, int]:
     """
     Pick (P, m) for a line of n points: P a multiple of 64, m even, P m >= n,
-    within the kernel limits (SH: m <= 8 -> P <= 512, m <= 16 -> P <= 256;
-    Line: P <= 512). Prefers m = 8, the
-    variant with the lighter register footprint, and the least padding.
+    within the kernel limits (m <= 8 -> P <= 512, m <= 16 -> P <= 256; Line:
+    P <= 512). SH radial lines: least padding, then the most rows per thread
+    (fewer, longer chunks: a shorter reduced system and more work per thread;
+    measured fastest on the 256-simulation scan). Line and cylindrical
+    z-lines: least padding, then m = 8, then the most rows.
     """
-    limits = {"sh": {8: 512, 16: 256}, "line": {8: 512, 16: 512}}[kind]
-    cands = [rows] if rows is not None else [8, 6, 4, 16, 14, 12, 10, 2]
+    limits = {"sh": {8: 512, 16: 256}, "cylz": {8: 512, 16: 256}, "line": {8: 512, 16: 512}}[kind]
+    cands = [rows] if rows is not None else [16, 14, 12, 10, 8, 6, 4, 2]
     best = None
     for m in cands:
         if m < 2 or m % 2:
@@ -53,7 +55,7 @@ def choose_layout(n: int, kind: str = "sh", rows: Optional[int] = None) -> Tuple
         P = 64 * -(-n // (64 * m))
         if P > cap:
             continue
-        key = (P * m - n, 0 if m == 8 else 1, -m)
+        key = (P * m - n, -m) if kind == "sh" else (P * m - n, 0 if m == 8 else 1, -m)
         if best is None or key < best[0]:
             best = (key, P, m)
     if best is None:
@@ -304,7 +306,7 @@ def choose_cyl_layout(z_points: int, rho_points: int, z_rows: Optional[int] = No
     z-lines per workgroup (Cr * lanes a multiple of 64, Cr <= 128); least rho
     padding first, then the order of CYL_RHO_SHAPES.
     """
-    Pz, Mz = choose_layout(z_points, "sh", z_rows)
+    Pz, Mz = choose_layout(z_points, "cylz", z_rows)
     best = None
     shapes = [tuple(rho_shape)] if rho_shape is not None else list(CYL_RHO_SHAPES)
     for pref, (mr, lanes) in enumerate(shapes):
