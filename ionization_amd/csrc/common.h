@@ -26,6 +26,17 @@ __device__ __forceinline__ void st2(double2* p, size_t i, double2 v) {
 }
 __device__ __forceinline__ double ld1(const double* p, size_t i) { return ((const TDSE_GLOBAL double*)p)[i]; }
 
+// Workgroups are dispatched round robin over the 8 XCDs (workgroup b runs on
+// XCD b % 8), each with its own L2. This bijection on [0, n) gives each XCD
+// one contiguous range of logical indices, so neighbouring logical work items
+// (which share cache lines) run on the same XCD at about the same time.
+#define TDSE_XCDS 8
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+    const int q = n / TDSE_XCDS, r = n % TDSE_XCDS;
+    const int xcd = b % TDSE_XCDS, slot = b / TDSE_XCDS;
+    return (xcd < r) ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
 // complex128 as double2 {re, im}
 __device__ __forceinline__ double2 cmk(double re, double im) { return make_double2(re, im); }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return cmk(a.x + b.x, a.y + b.y); }

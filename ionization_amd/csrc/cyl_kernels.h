@@ -163,9 +163,9 @@ template <int M, int LANES, int MODE, int MAXT, int WPE>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 8))) k_cyl_rho(CylSim* __restrict__ sims, const int2* __restrict__ groups,
                                                   int n_groups) {
     extern __shared__ double2 lds[];  // 3 * Cr * LANES double2, then 16 doubles for reductions
-    const int gi = blockIdx.x;
-    if (gi >= n_groups) return;
-    const int2 gt = groups[gi];
+    if ((int)blockIdx.x >= n_groups) return;
+    // neighbouring position groups share 128 B lines: keep them on one XCD
+    const int2 gt = groups[xcd_remap(blockIdx.x, n_groups)];
     CylSim& S = sims[gt.x];
     const int n = S.step;
     if (!S.active || n >= S.N) return;
