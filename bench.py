@@ -264,7 +264,8 @@ def _kernel_family(name):
     return base.split("<")[0]
 
 
-def summarize_profiles(d, command_trace, command_fetch, command_write):
+def summarize_profiles(d, command_trace, command_fetch, command_write, trace_steps=20, trace_warmup=5,
+                       trace_cyl_steps=20):
     """Reads rocprofv3 databases under d/{trace,fetch,write} and writes the
     kernel-time and HBM-traffic summaries under profiles/."""
     import glob
@@ -295,12 +296,29 @@ def summarize_profiles(d, command_trace, command_fetch, command_write):
         if tot < 1e5:
             continue
         lines.append(f"{name[:70]:<70} {cnt:>7d} {tot / 1e6:>10.3f} {avg / 1e3:>9.1f} {mn / 1e3:>9.1f} {mx / 1e3:>9.1f}")
-    lines += ["", "per kernel family (all template instances):",
-              f"{'family':<30} {'calls':>7} {'total ms':>10} {'avg us':>9}"]
+    # bench.py times every kernel in a profiled pass of direct launches, one kernel at a time,
+    # after the graph-replayed steps: those are the last dispatches of each family
+    starts = {}
+    for name, dur, st in con.execute("select name, duration, start from kernels order by start"):
+        starts.setdefault(_kernel_family(name), []).append(dur)
+    steps_of = {"k_sh_lphase": (trace_warmup + 2 * trace_steps, trace_steps),
+                "k_sh_rphase": (trace_warmup + 2 * trace_steps, trace_steps),
+                "k_cyl_z": (10 + 2 * trace_cyl_steps, trace_cyl_steps),
+                "k_cyl_rho": (10 + 2 * trace_cyl_steps, trace_cyl_steps)}
+    lines += ["", "per kernel family (all template instances); 'profiled pass' = the direct launches that",
+              "bench.py times with HIP events (its kernels.*.avg_ms and roofline.achieved):",
+              f"{'family':<30} {'calls':>7} {'total ms':>10} {'avg us':>9} {'profiled pass avg us':>22}"]
     for f, (c, t) in sorted(fam.items(), key=lambda kv: -kv[1][1]):
         if t < 1e5:
             continue
-        lines.append(f"{f:<30} {c:>7d} {t / 1e6:>10.3f} {t / c / 1e3:>9.1f}")
+        extra = ""
+        if f in steps_of and f in starts:
+            total_steps, prof_steps = steps_of[f]
+            per_step = len(starts[f]) // total_steps
+            last = starts[f][-per_step * prof_steps:] if per_step else []
+            if last:
+                extra = f"{np.mean(last) / 1e3:>22.1f}"
+        lines.append(f"{f:<30} {c:>7d} {t / 1e6:>10.3f} {t / c / 1e3:>9.1f}{extra}")
     with open(os.path.join(prof_dir, "sh_scan_kernel_trace.txt"), "w") as fh:
         fh.write("\n".join(lines) + "\n")
 
@@ -375,7 +393,8 @@ def main():
     ap.add_argument("--write-command", default="")
     args = ap.parse_args()
     if args.summarize_profiles:
-        summarize_profiles(args.summarize_profiles, args.trace_command, args.fetch_command, args.write_command)
+        summarize_profiles(args.summarize_profiles, args.trace_command, args.fetch_command, args.write_command,
+                           args.steps, args.warmup, args.cyl_steps)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (simulations shard by "
