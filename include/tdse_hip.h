@@ -94,8 +94,9 @@ typedef struct {
     int32_t store_every;   /* data cadence (sims.py:84-95); -1 = first and last only */
     int32_t n_basis;       /* test states for inner products */
     int32_t lb;            /* rows l < lb are captured at data steps (>= 1 + max basis l) */
-    double delta_r;        /* inner-product weight (meshes.py:1013) */
+    double delta_r;        /* inner-product weight (meshes.py:1013); r_j = (j + 1/2) delta_r */
     double corr0;          /* tau0 (-2 P_r) dr (1 + dr) / 8 (mesh_operators.py:901-903) */
+    double centrifugal;    /* tau0 hbar^2 / (2 mu): the U table is centrifugal / r_j^2 */
 } tdse_sh_desc;
 
 /*

@@ -31,6 +31,7 @@ class ShDesc(ctypes.Structure):
         ("lb", ctypes.c_int32),
         ("delta_r", ctypes.c_double),
         ("corr0", ctypes.c_double),
+        ("centrifugal", ctypes.c_double),
     ]
 
 

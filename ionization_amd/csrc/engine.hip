@@ -763,15 +763,19 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
     S.nwaves = (S.npad / TDSE_WAVE) * sh_segments(S.L);
     S.corr0 = D.corr0;
     S.delta_r = D.delta_r;
+    S.cu = D.centrifugal;
     S.step = 0;
     S.active = 1;
     H.n_data = n_data_points(S.N, S.store_every);
     const size_t npad = S.npad;
-    double2 *g, *g2, *snap, *oip;
+    double2 *g, *g2, *snap, *oip, *ta;
     double *tab, *dfs, *dts, *part, *on, *oln, *bas = nullptr;
     int* bl = nullptr;
     int rc = 0;
+    std::vector<double2> ta_host(npad);  // (T_j, A_j) interleaved for the R-phase
+    for (size_t p = 0; p < npad; ++p) ta_host[p] = make_double2(tables[SH_T * npad + p], tables[SH_AN * npad + p]);
     if ((rc = copy_to_device(ctx, H.blocks, reinterpret_cast<const double2*>(g0), (size_t)S.L * npad, &g)) ||
+        (rc = copy_to_device(ctx, H.blocks, ta_host.data(), npad, &ta)) ||
         (rc = copy_to_device<double2>(ctx, H.blocks, nullptr, (size_t)S.L * npad, &g2)) ||
         (rc = copy_to_device(ctx, H.blocks, tables, SH_NTAB * npad, &tab)) ||
         (rc = copy_to_device(ctx, H.blocks, fs, (size_t)S.N, &dfs)) ||
@@ -789,6 +793,7 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
     }
     S.g = g;
     S.g2 = g2;
+    S.ta = ta;
     S.tab = tab;
     S.fs = dfs;
     S.ts = dts;

@@ -40,8 +40,10 @@ struct ShSim {
     double2* out_ip;       // [n_data][n_basis]
     const double* basis;   // [n_basis][npad] radial g of each test state, position order
     const int* basis_l;    // [n_basis]
+    const double2* ta;     // [npad] (T_j, A_j) interleaved: the R-phase's only table read per row
     double corr0;          // tau0 * (-2 P_r) * dr (1 + dr) / 8, added at l = 0, j = 0
     double delta_r;
+    double cu;             // tau0 hbar^2 / (2 mu): U_j = cu / r_j^2, r_j = (j + 1/2) delta_r
     int n, L, P, m, npad, N, store_every, LB, n_basis, nwaves;
     int step;              // steps already advanced; device-updated
     int active;
@@ -351,29 +353,42 @@ k_sh_rphase(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_line
     const double ll1 = (double)l * (double)(l + 1);
     const double c0 = (l == 0 && k == 0) ? S.corr0 : 0.0;
     double2* __restrict__ gl = sh_state(S, n + 1) + (size_t)l * npad + k;
-    const double* __restrict__ T = S.tab + SH_T * npad + k;
-    const double* __restrict__ U = S.tab + SH_U * npad + k;
-    const double* __restrict__ AP = S.tab + SH_AP * npad + k;
-    const double* __restrict__ AN = S.tab + SH_AN * npad + k;
+    // One 16 B read per row and pass: (T_j, A_j). U_j = cu / r_j^2 is recomputed
+    // and the sub-diagonal A_{j-1} is carried from the previous row (the
+    // chunk's first row takes it from the neighbouring chunk's last row).
+    const double2* __restrict__ TA = S.ta + k;
+    const double dr = S.delta_r, cu = S.cu;
+    const int nr = S.n;
+    const double a_left = (k > 0) ? ld1(S.tab, SH_AN * npad + (M - 1) * P + k - 1) : 0.0;
 
     double2 x[M];
 #pragma unroll
     for (int i = 0; i < M; ++i) x[i] = ld2(gl, (size_t)i * P);
-    auto coef = [&](int i, double& hm, double& h, double& hp) {
-        h = sn * (fma(ll1, ld1(U, i * P), ld1(T, i * P)) + (i == 0 ? c0 : 0.0));
-        hm = sn * ld1(AP, i * P);
-        hp = sn * ld1(AN, i * P);
+    auto diag = [&](int i, double T) {
+        const int j = k * M + i;
+        const double r = fma((double)j, dr, 0.5 * dr);
+        const double U = (j < nr) ? cu * tdse_rcp(r * r) : 0.0;
+        return sn * (fma(ll1, U, T) + (i == 0 ? c0 : 0.0));
     };
-    // Second read of the same coefficients. The empty asm makes the pointers
-    // opaque so the compiler reloads instead of keeping the first pass's
-    // values live across the solver's barriers.
+    double carry = a_left;  // A_{j-1} of the next row (pass 1 visits rows in order)
+    auto coef = [&](int i, double& hm, double& h, double& hp) {
+        const double2 tv = ld2(TA, (size_t)i * P);
+        h = diag(i, tv.x);
+        hm = sn * carry;
+        hp = sn * tv.y;
+        carry = tv.y;
+    };
+    // Second read of the same coefficients (pass 2, rows in order). The empty
+    // asm makes the pointer opaque so the compiler reloads instead of keeping
+    // the first pass's values live across the solver's barriers.
+    double carry2 = a_left;
     auto coef2 = [&](int i, double& hm, double& h) {
-        const double* T2 = T;
-        const double* U2 = U;
-        const double* AP2 = AP;
-        asm volatile("" : "+v"(T2), "+v"(U2), "+v"(AP2));
-        h = sn * (fma(ll1, ld1(U2, i * P), ld1(T2, i * P)) + (i == 0 ? c0 : 0.0));
-        hm = sn * ld1(AP2, i * P);
+        const double2* TA2 = TA;
+        asm volatile("" : "+v"(TA2));
+        const double2 tv = ld2(TA2, (size_t)i * P);
+        h = diag(i, tv.x);
+        hm = sn * carry2;
+        carry2 = tv.y;
     };
     cn_line_finish<M, FORM>(x, gl, k, P, coef, coef2, lds);
 }

@@ -119,6 +119,7 @@ class CompiledSH:
     lb: int
     delta_r: float
     corr0: float
+    centrifugal: float  # tau0 hbar^2 / (2 mu); U_j = centrifugal / r_j^2
     tables: np.ndarray  # (6, npad)
     fs: np.ndarray  # (N,)
     ts: np.ndarray  # (N,)
@@ -159,7 +160,8 @@ def sh_radial_tables(*, r_bound, r_points, potential, mask, time_step, test_char
       RQ_j = tau0 (-q) r_j                                 (mesh_operators.py:988-1006)
       MASK_j = mask(r_j)                                   (masks.py:76-89)
     with P_r = -hbar^2 / (2 mu dr^2), the reduced mass mu regardless of test_mass
-    (mesh_operators.py:892-894). Returns (tables, corr0, delta_r, r).
+    (mesh_operators.py:892-894). Returns (tables, corr0, delta_r, r, centrifugal) with
+    centrifugal = tau0 hbar^2 / (2 mu), from which the R-phase recomputes U_j.
     """
     r = np.linspace(0, r_bound, r_points)
     delta_r = r[1] - r[0]
@@ -174,7 +176,8 @@ def sh_radial_tables(*, r_bound, r_points, potential, mask, time_step, test_char
     alpha = (xa + 1) / (xa + 0.75)
     v = np.asarray(potential(r), dtype=np.float64) * np.ones_like(r)
     T = tau0 * (beta * (-2 * pr) + v)
-    U = tau0 * ((u.hbar ** 2) / (2 * mu)) / (r ** 2)
+    centrifugal = tau0 * ((u.hbar ** 2) / (2 * mu))
+    U = centrifugal / (r ** 2)
     A = tau0 * pr * alpha
     A[-1] = 0.0
     AP = np.concatenate([[0.0], A[:-1]])
@@ -185,7 +188,7 @@ def sh_radial_tables(*, r_bound, r_points, potential, mask, time_step, test_char
         dr = delta_r / u.bohr_radius
         corr0 = tau0 * (-2 * pr) * (dr * (1 + dr) / 8)
     tables = to_positions(np.stack([T, U, AP, A, RQ, MK]), P, m)
-    return tables, corr0, delta_r, r
+    return tables, corr0, delta_r, r, centrifugal
 
 
 def compile_sh(*, r_bound, r_points, l_bound, potential, field: Callable, mask, times, time_step,
@@ -198,7 +201,7 @@ def compile_sh(*, r_bound, r_points, l_bound, potential, field: Callable, mask, 
     t_{n+1} + time_step / 2 (mesh_operators.py:1011-1013).
     """
     P, m = choose_layout(r_points, "sh", rows)
-    tables, corr0, delta_r, _ = sh_radial_tables(
+    tables, corr0, delta_r, _, centrifugal = sh_radial_tables(
         r_bound=r_bound, r_points=r_points, potential=potential, mask=mask, time_step=time_step,
         test_charge=test_charge, hydrogen_correction=hydrogen_correction, P=P, m=m,
     )
@@ -217,7 +220,8 @@ def compile_sh(*, r_bound, r_points, l_bound, potential, field: Callable, mask, 
         bl[i] = l
     return CompiledSH(
         r_points=r_points, l_bound=l_bound, P=P, m=m, n_steps=len(steps), store_every=int(store_every),
-        lb=min(lb, l_bound), delta_r=delta_r, corr0=corr0, tables=np.ascontiguousarray(tables),
+        lb=min(lb, l_bound), delta_r=delta_r, corr0=corr0, centrifugal=float(centrifugal),
+        tables=np.ascontiguousarray(tables),
         fs=np.ascontiguousarray(fs), ts=np.ascontiguousarray(ts),
         g0=np.ascontiguousarray(to_positions(g0, P, m)),
         basis=np.ascontiguousarray(to_positions(bmat, P, m)), basis_l=bl,
@@ -518,7 +522,7 @@ class DeviceContext:
     # -- simulations
     def add_sh(self, c: CompiledSH) -> int:
         desc = _lib.ShDesc(c.r_points, c.l_bound, c.P, c.m, c.n_steps, c.store_every, len(c.basis_l), c.lb,
-                           c.delta_r, c.corr0)
+                           c.delta_r, c.corr0, c.centrifugal)
         slot = ctypes.c_int32()
         basis = np.ascontiguousarray(c.basis, dtype=np.float64) if len(c.basis_l) else None
         bl = np.ascontiguousarray(c.basis_l, dtype=np.int32)
