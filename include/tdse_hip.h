@@ -158,6 +158,8 @@ typedef struct {
     double test_charge;
     double delta;                  /* inner-product weight dz drho (meshes.py:541) */
     double mask_inner, mask_outer, mask_smoothness;  /* RadialCosineMask (potentials/masks.py:38-89) */
+    double z_first, z_step;        /* z_j = z_first + j z_step (the kernels recompute the grid) */
+    double rho_first, rho_step;    /* rho_j = rho_first + j rho_step */
 } tdse_cyl_desc;
 
 /*

@@ -71,6 +71,10 @@ class CylDesc(ctypes.Structure):
         ("mask_inner", ctypes.c_double),
         ("mask_outer", ctypes.c_double),
         ("mask_smoothness", ctypes.c_double),
+        ("z_first", ctypes.c_double),
+        ("z_step", ctypes.c_double),
+        ("rho_first", ctypes.c_double),
+        ("rho_step", ctypes.c_double),
     ]
 
 

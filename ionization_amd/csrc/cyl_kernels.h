@@ -48,6 +48,7 @@ struct CylSim {
     double q;               // test charge
     double delta;           // inner-product weight dz drho
     double m_inner, m_outer, m_inv_s;  // RadialCosineMask (inner, outer, 1 / smoothness)
+    double z0, dz, rho0, drho;         // grids: z_j = z0 + j dz, rho_j = rho0 + j drho
     int has_mask;
     int nz, nr, Pz, Mz, npz, Cr, Mr, nrpad, lanes, n_groups;
     int N, store_every, n_basis;
@@ -72,6 +73,14 @@ __device__ __forceinline__ double cyl_rsqrt(double x) {
 // internal_hamiltonian adds V / 2, total_hamiltonian adds (-z q E) / 2.
 __device__ __forceinline__ double cyl_half_potential(const CylSim& S, double z, double rho, double E) {
     return 0.5 * (S.kc * cyl_rsqrt(z * z + rho * rho)) + 0.5 * (-z * S.q * E);
+}
+
+// Off-diagonal of H_rho between rho indices j and j + 1: rp c(j + 1),
+// c(m) = m / sqrt(m^2 - 1/4) (mesh_operators.py:582-592); 0 past the line.
+__device__ __forceinline__ double cyl_rho_off(const CylSim& S, int j) {
+    if (j < 0 || j + 1 >= S.nr) return 0.0;
+    const double m = (double)(j + 1);
+    return (m * cyl_rsqrt(m * m - 0.25)) * S.rp;
 }
 
 // RadialCosineMask.__call__ (potentials/masks.py:76-89) at r.
@@ -104,31 +113,27 @@ k_cyl_z(CylSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines) 
     const int k = threadIdx.x;
     const double tau = ld1(S.ts, n);
     const double E = ld1(S.fs, n);
-    const double rho = ld1(S.rt, lt.y);
+    const double rho = fma((double)lt.y, S.drho, S.rho0);
     const double tzp = tau * S.zp;
     const double tdiag = tau * (S.zp * -2.0);
     double2* __restrict__ gl = S.g + (size_t)lt.y * S.npz + k;
-    const double* __restrict__ zt = S.zt + k;
+    const double z0 = S.z0, dz = S.dz;
 
     double2 x[M];
 #pragma unroll
     for (int i = 0; i < M; ++i) x[i] = ld2(gl, (size_t)i * P);
+    // coefficients from the grid, no table reads
     auto coef = [&](int i, double& hm, double& h, double& hp) {
         const int j = k * M + i;
         const bool live = j < nz;
-        const double z = ld1(zt, i * P);
+        const double z = fma((double)j, dz, z0);
         h = live ? fma(tau, cyl_half_potential(S, z, rho, E), tdiag) : 0.0;
         hm = (live && j >= 1) ? tzp : 0.0;
         hp = (j + 1 < nz) ? tzp : 0.0;
     };
     auto coef2 = [&](int i, double& hm, double& h) {
-        const double* zt2 = zt;
-        asm volatile("" : "+v"(zt2));
-        const int j = k * M + i;
-        const bool live = j < nz;
-        const double z = ld1(zt2, i * P);
-        h = live ? fma(tau, cyl_half_potential(S, z, rho, E), tdiag) : 0.0;
-        hm = (live && j >= 1) ? tzp : 0.0;
+        double hp;
+        coef(i, hm, h, hp);
     };
     cn_line_finish<M, FORM>(x, gl, k, P, coef, coef2, lds);
 }
@@ -184,27 +189,29 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 
 #pragma unroll
     for (int i = 0; i < M; ++i) x[i] = ld2(gl, (size_t)i * npz);
 
-    // rho-direction coefficients of row i (rho index j = c M + i) for (tau, E);
-    // rt / roff are passed in so a second pass can hand in opaque copies.
-    auto rho_coef = [&](const double* rt, const double* roff, double tau, double E, int i, double& hm, double& h,
-                        double& hp) {
+    // rho-direction coefficients of row i (rho index j = c M + i) for (tau, E),
+    // from the grid: rho_j recomputed, the sub-diagonal carried from the
+    // previous row (callers visit rows in order, starting from `carry` =
+    // cyl_rho_off(j0 - 1)).
+    const double a_left = line_live ? cyl_rho_off(S, c * M - 1) : 0.0;
+    auto rho_coef = [&](double& carry, double tau, double E, int i, double& hm, double& h, double& hp) {
         const int j = c * M + i;
         const bool live = line_live && j < nr;
-        const double rho = ld1(rt, j);
+        const double rho = fma((double)j, S.drho, S.rho0);
+        const double off = live ? cyl_rho_off(S, j) : 0.0;
         h = live ? fma(tau, cyl_half_potential(S, z, rho, E), tau * (S.rp * -2.0)) : 0.0;
-        hm = (live && j >= 1) ? tau * ld1(roff, j >= 1 ? j - 1 : 0) : 0.0;
-        hp = live ? tau * ld1(roff, j) : 0.0;
+        hm = (live && j >= 1) ? tau * carry : 0.0;
+        hp = tau * off;
+        carry = off;
     };
 
     if constexpr (MODE == CYL_MAIN) {
         const double tau = ld1(S.ts, n), E = ld1(S.fs, n);
-        auto coef = [&](int i, double& hm, double& h, double& hp) { rho_coef(S.rt, S.roff, tau, E, i, hm, h, hp); };
+        double carry1 = a_left, carry2 = a_left;
+        auto coef = [&](int i, double& hm, double& h, double& hp) { rho_coef(carry1, tau, E, i, hm, h, hp); };
         auto coef2 = [&](int i, double& hm, double& h) {
-            const double* rt2 = S.rt;
-            const double* roff2 = S.roff;
-            asm volatile("" : "+v"(rt2), "+v"(roff2));
             double hp;
-            rho_coef(rt2, roff2, tau, E, i, hm, h, hp);
+            rho_coef(carry2, tau, E, i, hm, h, hp);
         };
         cn_implicit_lowreg<M, decltype(coef), decltype(coef2), false, LANES>(x, c, P, coef, coef2, lds,
                                                                             cmk(0.0, 0.0), cmk(0.0, 0.0), ln);
@@ -212,7 +219,7 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 
 #pragma unroll
             for (int i = 0; i < M; ++i) {
                 const int j = c * M + i;
-                const double mk = (line_live && j < nr) ? cyl_mask(S, z, ld1(S.rt, j)) : 0.0;
+                const double mk = (line_live && j < nr) ? cyl_mask(S, z, fma((double)j, S.drho, S.rho0)) : 0.0;
                 x[i] = cscale(mk, x[i]);
             }
         }
@@ -254,10 +261,11 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 
         double2 xl, xr;
         chunk_neighbours<M, LANES>(x, c, P, lds, xl, xr, true, ln);
         double2 prev = xl;
+        double carry3 = a_left;
 #pragma unroll
         for (int i = 0; i < M; ++i) {
             double hm, h, hp;
-            rho_coef(S.rt, S.roff, tau, E, i, hm, h, hp);
+            rho_coef(carry3, tau, E, i, hm, h, hp);
             const double2 xp = (i + 1 < M) ? x[i + 1 < M ? i + 1 : i] : xr;
             const double2 hx = cmk(fma(hm, prev.x, fma(h, x[i].x, hp * xp.x)), fma(hm, prev.y, fma(h, x[i].y, hp * xp.y)));
             prev = x[i];

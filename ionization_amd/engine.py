@@ -368,6 +368,10 @@ class CompiledCyl:
     test_charge: float
     delta: float
     mask: Optional[Tuple[float, float, float]]
+    z_first: float
+    z_step: float
+    rho_first: float
+    rho_step: float
     zt: np.ndarray  # (npz,)
     rt: np.ndarray  # (nrpad,)
     roff: np.ndarray  # (nrpad,)
@@ -435,6 +439,7 @@ def compile_cyl(*, z_bound, rho_bound, z_points, rho_points, field: Callable, ti
         n_steps=len(steps), store_every=int(store_every), z_prefactor=float(zp), rho_prefactor=float(rp),
         coulomb=float(u.coulomb_constant * charge * test_charge), test_charge=float(test_charge),
         delta=float(delta_z * delta_rho), mask=None if mask is None else tuple(float(v) for v in mask),
+        z_first=float(z[0]), z_step=float(delta_z), rho_first=float(rho[0]), rho_step=float(delta_rho),
         zt=np.ascontiguousarray(zt), rt=rt, roff=roff, fs=np.ascontiguousarray(fs), ts=np.ascontiguousarray(ts),
         g0=np.ascontiguousarray(to_cyl_positions(np.asarray(g0, np.complex128), Pz, Mz, nrpad)),
         basis=np.ascontiguousarray(to_cyl_positions(bmat, Pz, Mz, nrpad)),
@@ -547,7 +552,8 @@ class DeviceContext:
         mk = c.mask if c.mask is not None else (0.0, 0.0, 1.0)
         desc = _lib.CylDesc(c.z_points, c.rho_points, c.Pz, c.Mz, c.Cr, c.Mr, c.lanes, c.n_steps, c.store_every,
                             len(c.basis), 0 if c.mask is None else 1, 0, c.z_prefactor, c.rho_prefactor, c.coulomb,
-                            c.test_charge, c.delta, mk[0], mk[1], mk[2])
+                            c.test_charge, c.delta, mk[0], mk[1], mk[2], c.z_first, c.z_step, c.rho_first,
+                            c.rho_step)
         slot = ctypes.c_int32()
         basis = np.ascontiguousarray(c.basis, dtype=np.complex128) if len(c.basis) else None
         self._check(self._lib.tdse_cyl_add(
