@@ -127,6 +127,9 @@ typedef struct {
     int32_t renormalize;   /* 1: renormalise after every step (imaginary-time relaxation) */
     int32_t reserved;
     double delta_z;
+    double offdiag;        /* tau0 P_z, the off-diagonal inside the line (tables 1 and 2 hold it too) */
+    double zq_first;       /* tau0 (-q) z_0: the z coupling of row j is zq_first + j zq_step */
+    double zq_step;
 } tdse_line_desc;
 
 int tdse_line_add(tdse_ctx *ctx, const tdse_line_desc *desc, const double *tables, const double *wt,

@@ -46,6 +46,9 @@ class LineDesc(ctypes.Structure):
         ("renormalize", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
         ("delta_z", ctypes.c_double),
+        ("offdiag", ctypes.c_double),
+        ("zq_first", ctypes.c_double),
+        ("zq_step", ctypes.c_double),
     ]
 
 

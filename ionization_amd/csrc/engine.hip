@@ -854,6 +854,9 @@ int tdse_line_add(tdse_ctx* ctx, const tdse_line_desc* desc, const double* table
     S.n_basis = D.n_basis;
     S.renormalize = D.renormalize;
     S.delta_z = D.delta_z;
+    S.off = D.offdiag;
+    S.zq0 = D.zq_first;
+    S.dzq = D.zq_step;
     S.step = 0;
     S.active = 1;
     H.n_data = n_data_points(S.N, S.store_every);

@@ -239,6 +239,9 @@ class CompiledLine:
     store_every: int
     delta_z: float
     renormalize: bool
+    offdiag: float  # tau0 P_z
+    zq_first: float  # tau0 (-q) z_0
+    zq_step: float  # tau0 (-q) dz
     tables: np.ndarray  # (5, npad)
     wt: np.ndarray  # (N,) complex: i tau_n / tau0
     fs: np.ndarray  # (N,)
@@ -291,7 +294,9 @@ def compile_line(*, z_bound, z_points, potential, field: Callable, mask, times, 
     bmat = np.array([np.asarray(b, dtype=np.complex128) for b in basis]).reshape(len(basis), z_points)
     return CompiledLine(
         z_points=z_points, P=P, m=m, n_steps=len(steps), store_every=int(store_every), delta_z=delta_z,
-        renormalize=bool(renormalize), tables=np.ascontiguousarray(tables), wt=np.ascontiguousarray(wt),
+        renormalize=bool(renormalize), offdiag=float(tau0 * pz), zq_first=float(ZQ[0]),
+        zq_step=float(tau0 * (-test_charge) * (z[1] - z[0])),
+        tables=np.ascontiguousarray(tables), wt=np.ascontiguousarray(wt),
         fs=np.ascontiguousarray(fs), g0=np.ascontiguousarray(to_positions(np.asarray(g0, np.complex128), P, m)),
         basis=np.ascontiguousarray(to_positions(bmat, P, m)),
     )
@@ -539,7 +544,7 @@ class DeviceContext:
 
     def add_line(self, c: CompiledLine) -> int:
         desc = _lib.LineDesc(c.z_points, c.P, c.m, c.n_steps, c.store_every, len(c.basis), int(c.renormalize), 0,
-                             c.delta_z)
+                             c.delta_z, c.offdiag, c.zq_first, c.zq_step)
         slot = ctypes.c_int32()
         basis = np.ascontiguousarray(c.basis, dtype=np.complex128) if len(c.basis) else None
         self._check(self._lib.tdse_line_add(
