@@ -84,8 +84,7 @@ __device__ __forceinline__ double cyl_rho_off(const CylSim& S, int j) {
 }
 
 // RadialCosineMask.__call__ (potentials/masks.py:76-89) at r.
-__device__ __forceinline__ double cyl_mask(const CylSim& S, double z, double rho) {
-    const double r = sqrt(z * z + rho * rho);
+__device__ __forceinline__ double cyl_mask_r(const CylSim& S, double r) {
     if (r < S.m_inner) return 1.0;
     if (r >= S.m_outer) return 0.0;
     double sn, cs;
@@ -194,12 +193,38 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 
     // previous row (callers visit rows in order, starting from `carry` =
     // cyl_rho_off(j0 - 1)).
     const double a_left = line_live ? cyl_rho_off(S, c * M - 1) : 0.0;
+    // With few rows per thread, half the Coulomb potential (0.5 k Q q / r) and r
+    // (for the mask) are computed once per row, from one refined reciprocal
+    // square root, and kept; with more rows they are recomputed where needed.
+    constexpr bool KEEP = M <= 4;
+    double vhalf[KEEP ? M : 1], rad[KEEP ? M : 1];
+    auto rho_of = [&](int i) { return fma((double)(c * M + i), S.drho, S.rho0); };
+    if constexpr (KEEP) {
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const int j = c * M + i;
+            const double rho = rho_of(i);
+            const double r2 = z * z + rho * rho;
+            const double ri = cyl_rsqrt(r2);
+            vhalf[i] = (line_live && j < nr) ? 0.5 * (S.kc * ri) : 0.0;
+            rad[i] = r2 * ri;
+        }
+    }
+    auto half_v = [&](int i) {
+        if constexpr (KEEP) return vhalf[i < M ? i : 0];
+        const double rho = rho_of(i);
+        return 0.5 * (S.kc * cyl_rsqrt(z * z + rho * rho));
+    };
+    auto radius = [&](int i) {
+        if constexpr (KEEP) return rad[i < M ? i : 0];
+        const double rho = rho_of(i);
+        return sqrt(z * z + rho * rho);
+    };
     auto rho_coef = [&](double& carry, double tau, double E, int i, double& hm, double& h, double& hp) {
         const int j = c * M + i;
         const bool live = line_live && j < nr;
-        const double rho = fma((double)j, S.drho, S.rho0);
         const double off = live ? cyl_rho_off(S, j) : 0.0;
-        h = live ? fma(tau, cyl_half_potential(S, z, rho, E), tau * (S.rp * -2.0)) : 0.0;
+        h = live ? fma(tau, half_v(i) + 0.5 * (-z * S.q * E), tau * (S.rp * -2.0)) : 0.0;
         hm = (live && j >= 1) ? tau * carry : 0.0;
         hp = tau * off;
         carry = off;
@@ -219,7 +244,7 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 
 #pragma unroll
             for (int i = 0; i < M; ++i) {
                 const int j = c * M + i;
-                const double mk = (line_live && j < nr) ? cyl_mask(S, z, fma((double)j, S.drho, S.rho0)) : 0.0;
+                const double mk = (line_live && j < nr) ? cyl_mask_r(S, radius(i)) : 0.0;
                 x[i] = cscale(mk, x[i]);
             }
         }
