@@ -97,6 +97,8 @@ typedef struct {
     double delta_r;        /* inner-product weight (meshes.py:1013); r_j = (j + 1/2) delta_r */
     double corr0;          /* tau0 (-2 P_r) dr (1 + dr) / 8 (mesh_operators.py:901-903) */
     double centrifugal;    /* tau0 hbar^2 / (2 mu): the U table is centrifugal / r_j^2 */
+    int32_t norm_by_l;     /* 1: also reduce the norm in every l at data times (NormBySphericalHarmonic) */
+    int32_t reserved;
 } tdse_sh_desc;
 
 /*
@@ -221,6 +223,15 @@ int tdse_download(tdse_ctx *ctx, int32_t kind, int32_t slot, double *g, double *
  * else returns TDSE_ERR_STATE.
  */
 int tdse_materialize(tdse_ctx *ctx, int32_t kind, int32_t slot);
+
+/*
+ * The norm in every spherical harmonic l at every data time, n_data * L
+ * doubles (data index major), for an SH simulation added with norm_by_l = 1.
+ * Replaces SphericalHarmonicMesh.norm_by_l as stored by the
+ * NormBySphericalHarmonic datastore (ionization/mesh/meshes.py:1130-1133,
+ * data.py:438-458).
+ */
+int tdse_download_norm_by_l(tdse_ctx *ctx, int32_t slot, double *out);
 
 /*
  * Overwrite a simulation's wavefunction (position order; the reference g after

@@ -32,6 +32,8 @@ class ShDesc(ctypes.Structure):
         ("delta_r", ctypes.c_double),
         ("corr0", ctypes.c_double),
         ("centrifugal", ctypes.c_double),
+        ("norm_by_l", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -119,6 +121,7 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32] + [c_double_p] * 4,
     ),
+    "tdse_download_norm_by_l": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_double_p]),
     "tdse_upload_state": (
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, c_double_p, ctypes.c_int32],

@@ -136,7 +136,7 @@ struct Bucket {
 };
 
 // l segments of at most kSegRows rows per L-phase wave, balanced in size.
-constexpr int kSegRows = 64;
+constexpr int kSegRows = kShSegRows;
 int sh_segments(int L) { return (L + kSegRows - 1) / kSegRows; }
 
 int n_data_points(int N, int every) {
@@ -169,6 +169,7 @@ struct tdse_ctx {
     hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
     std::vector<Bucket> line_buckets;
     bool dirty = true;
+    bool sh_norm_by_l = false;   // some live SH simulation reduces the norm in every l (L-phase instance)
     bool cyl_pending = false;    // a cylindrical simulation was added or uploaded: it starts from an exact state
     bool graph_cyl_pre = false;  // the captured step includes the cylindrical pre-sweep
     hipGraph_t graph = nullptr;
@@ -223,6 +224,7 @@ int rebuild_tables(tdse_ctx* ctx) {
     std::vector<ShWave> waves;
     std::vector<int2> jobs;
     std::map<std::pair<int, int>, std::vector<int2>> shb, lnb;  // (rows, chunks [+ 4096 if complex steps])
+    ctx->sh_norm_by_l = false;
     for (int s = 0; s < (int)ctx->sh.size(); ++s) {
         ShHost& h = ctx->sh[s];
         if (!h.live) continue;
@@ -236,6 +238,10 @@ int rebuild_tables(tdse_ctx* ctx) {
         }
         h.wave_count = (int)waves.size() - h.wave_first;
         jobs.push_back(make_int2(s, -1));
+        if (h.dev.norm_by_l) {
+            jobs.push_back(make_int2(s, -2));
+            ctx->sh_norm_by_l = true;
+        }
         for (int b = 0; b < h.dev.n_basis; ++b) jobs.push_back(make_int2(s, b));
         auto& v = shb[{h.dev.m, h.dev.P}];
         for (int l = 0; l < h.dev.L; ++l) v.push_back(make_int2(s, l));
@@ -445,7 +451,9 @@ int launch_step(tdse_ctx* ctx, bool cyl_pre) {
     const int nw = (int)ctx->waves.count;
     if (nw > 0) {
         prof_begin(ctx, 0);
-        hipLaunchKernelGGL(k_sh_lphase, dim3((nw * TDSE_WAVE + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh,
+        // the instance that reduces the norm in every l needs more registers: only when asked for
+        auto lphase = ctx->sh_norm_by_l ? k_sh_lphase<true> : k_sh_lphase<false>;
+        hipLaunchKernelGGL(lphase, dim3((nw * TDSE_WAVE + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh,
                            ctx->waves.ptr, nw, ctx->d_cl, 0);
         prof_end(ctx);
         const int nj = (int)ctx->jobs.count;
@@ -761,6 +769,7 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
     S.LB = D.lb;
     S.n_basis = D.n_basis;
     S.nwaves = (S.npad / TDSE_WAVE) * sh_segments(S.L);
+    S.norm_by_l = D.norm_by_l ? 1 : 0;
     S.corr0 = D.corr0;
     S.delta_r = D.delta_r;
     S.cu = D.centrifugal;
@@ -769,7 +778,7 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
     H.n_data = n_data_points(S.N, S.store_every);
     const size_t npad = S.npad;
     double2 *g, *g2, *snap, *oip, *ta;
-    double *tab, *dfs, *dts, *part, *on, *oln, *bas = nullptr;
+    double *tab, *dfs, *dts, *part, *on, *oln, *bas = nullptr, *pl = nullptr, *onl = nullptr;
     int* bl = nullptr;
     int rc = 0;
     std::vector<double2> ta_host(npad);  // (T_j, A_j) interleaved for the R-phase
@@ -786,7 +795,10 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
         (rc = copy_to_device<double>(ctx, H.blocks, nullptr, (size_t)H.n_data, &oln)) ||
         (rc = copy_to_device<double2>(ctx, H.blocks, nullptr, (size_t)H.n_data * std::max(1, S.n_basis), &oip)) ||
         (S.n_basis > 0 && ((rc = copy_to_device(ctx, H.blocks, basis, (size_t)S.n_basis * npad, &bas)) ||
-                           (rc = copy_to_device(ctx, H.blocks, basis_l, (size_t)S.n_basis, &bl))))) {
+                           (rc = copy_to_device(ctx, H.blocks, basis_l, (size_t)S.n_basis, &bl)))) ||
+        (S.norm_by_l &&
+         ((rc = copy_to_device<double>(ctx, H.blocks, nullptr, (size_t)S.nwaves * kShSegRows, &pl)) ||
+          (rc = copy_to_device<double>(ctx, H.blocks, nullptr, (size_t)H.n_data * S.L, &onl))))) {
         (void)hipStreamSynchronize(ctx->stream);
         free_blocks(ctx, H.blocks);
         return rc;
@@ -804,6 +816,8 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
     S.out_ip = oip;
     S.basis = bas;
     S.basis_l = bl;
+    S.part_l = pl;
+    S.out_nl = onl;
     HIP_TRY(ctx, hipMemcpyAsync(ctx->d_sh + s, &S, sizeof(ShSim), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // host buffers may be released after return
     H.live = true;
@@ -1139,6 +1153,17 @@ int tdse_download(tdse_ctx* ctx, int32_t kind, int32_t slot, double* g, double* 
     return TDSE_OK;
 }
 
+int tdse_download_norm_by_l(tdse_ctx* ctx, int32_t slot, double* out) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    if (!out) return fail(ctx, TDSE_ERR_ARG, "out is NULL");
+    if (slot < 0 || slot >= (int)ctx->sh.size() || !ctx->sh[slot].live) return fail(ctx, TDSE_ERR_ARG, "bad SH slot");
+    const ShHost& H = ctx->sh[slot];
+    if (!H.dev.norm_by_l) return fail(ctx, TDSE_ERR_STATE, "simulation was added without norm_by_l");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipMemcpy(out, H.dev.out_nl, (size_t)H.n_data * H.dev.L * sizeof(double), hipMemcpyDeviceToHost));
+    return TDSE_OK;
+}
+
 int tdse_upload_state(tdse_ctx* ctx, int32_t kind, int32_t slot, const double* g, int32_t step) {
     if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
     if (!g) return fail(ctx, TDSE_ERR_ARG, "g is NULL");
@@ -1239,8 +1264,8 @@ int tdse_materialize(tdse_ctx* ctx, int32_t kind, int32_t slot) {
     const ShHost& H = ctx->sh[slot];
     if (st[1] == 1 || st[0] == 0 || st[0] > H.dev.N) return TDSE_OK;
     const int nw = H.wave_count;
-    hipLaunchKernelGGL(k_sh_lphase, dim3((nw * TDSE_WAVE + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh,
-                       ctx->waves.ptr + H.wave_first, nw, ctx->d_cl, 1);
+    hipLaunchKernelGGL(k_sh_lphase<false>, dim3((nw * TDSE_WAVE + 255) / 256), dim3(256), 0, ctx->stream,
+                       ctx->d_sh, ctx->waves.ptr + H.wave_first, nw, ctx->d_cl, 1);
     HIP_TRY(ctx, hipGetLastError());
     // the end-only L-phase wrote sh_state(n + 1): swap the buffers so the exact
     // state is sh_state(n) for the unchanged step counter

@@ -38,6 +38,8 @@ struct ShSim {
     double* out_norm;      // [n_data]
     double* out_lnorm;     // [n_data]
     double2* out_ip;       // [n_data][n_basis]
+    double* part_l;        // [nwaves][kShSegRows] per-wave |g|^2 sums of each row (norm by l), or null
+    double* out_nl;        // [n_data][L] norm in each l, or null
     const double* basis;   // [n_basis][npad] radial g of each test state, position order
     const int* basis_l;    // [n_basis]
     const double2* ta;     // [npad] (T_j, A_j) interleaved: the R-phase's only table read per row
@@ -48,7 +50,10 @@ struct ShSim {
     int step;              // steps already advanced; device-updated
     int active;
     int phase;             // 0: g is mid-step (after the R-phase); 1: g is the reference g after `step` steps
+    int norm_by_l;         // 1: also reduce the norm of every l at data times
 };
+
+constexpr int kShSegRows = 64;  // most rows of one L-phase wave (its l-segment)
 
 enum { SH_T = 0, SH_U = 1, SH_AP = 2, SH_AN = 3, SH_RQ = 4, SH_MASK = 5, SH_NTAB = 6 };
 
@@ -105,6 +110,9 @@ struct ShWave {
     int sim, p0, lo, hi, part;  // part: index of this wave's norm partial within the simulation
 };
 
+// BY_L: the instance that can also reduce the norm in every l (launched only
+// while some simulation asks for it: the reductions cost registers).
+template <bool BY_L>
 __global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, const ShWave* __restrict__ waves,
                                                    int n_waves, const double* __restrict__ cl, int end_only) {
     constexpr int PF = 4;  // prefetch depth in iterations (2 rows each)
@@ -148,12 +156,19 @@ __global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, con
     };
 
     // first iteration: rows below lo - 4 do not influence rows >= lo
-    const int it0 = (lo <= 4) ? 0 : ((lo + q - 4) >> 1);
-    const int it_end = (hi - 1 + q + 3) >> 1;  // last iteration stores row hi-1
+    // Rows of one lane's iteration depend on its parity q, so with odd L the
+    // lanes of a wave may have different bounds. Reducing the norm of every
+    // row across the wave (norm by l) needs the same bounds for all lanes: the
+    // union is used then (an earlier start or later end only adds pipeline
+    // warm-up / run-out on rows that are not stored).
+    const bool by_l = BY_L && observe && S.norm_by_l;
+    const int it0 = (lo <= 4) ? 0 : ((lo + (by_l ? 0 : q) - 4) >> 1);
+    const int it_end = (hi - 1 + (by_l ? 1 : q) + 3) >> 1;  // last iteration stores row hi-1
     const double2 zero = make_double2(0.0, 0.0);
     double2 carry_u = zero, carry_s1 = zero, carry_s3 = zero;
     double c_m3 = load_c(2 * it0 - 3), c_m2 = load_c(2 * it0 - 2), c_m1 = load_c(2 * it0 - 1);
     double acc_norm = 0.0, acc_lnorm = 0.0;
+    double pend = 0.0;  // norm by l: the q = 0 lanes' share of odd row v0 - 1, completed next iteration
     double2 ring0[PF], ring1[PF];
     double ringc0[PF], ringc1[PF];
 #pragma unroll
@@ -208,6 +223,21 @@ __global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, con
                     if (le + 1 < S.LB) st2(S.snap, (size_t)(le + 1) * npad + p, s2b);
                 }
             }
+            if (by_l) {
+                // this iteration finishes rows v0-2 (all lanes), v0-1 (q = 0 lanes)
+                // and v0-3 (q = 1 lanes); sums in a fixed order per wave
+                const bool ok_a = live && le >= lo && le < hi, ok_b = live && le + 1 >= lo && le + 1 < hi;
+                const double na = ok_a ? cabs2(s2a) : 0.0, nb = ok_b ? cabs2(s2b) : 0.0;
+                const double r3 = wave_sum(q ? na : 0.0);  // row v0-3
+                const double r2 = wave_sum(q ? nb : na);   // row v0-2
+                const double r1 = wave_sum(q ? 0.0 : nb);  // row v0-1
+                if (lane == 0) {
+                    double* pl = S.part_l + (size_t)wt.part * kShSegRows;
+                    if (v0 - 3 >= lo && v0 - 3 < hi) pl[v0 - 3 - lo] = pend + r3;
+                    if (v0 - 2 >= lo && v0 - 2 < hi) pl[v0 - 2 - lo] = r2;
+                }
+                pend = r1;
+            }
             if (do_start) sh_rotate(f_start * c_m2, s2a, s2b);
         }
         // stage 4: O_start on (v0-3, v0-2)
@@ -255,6 +285,21 @@ __global__ void __launch_bounds__(256) k_sh_observe(ShSim* __restrict__ sims, co
     __shared__ double red_im[256];
     const int t = threadIdx.x;
     double re = 0.0, im = 0.0;
+    if (jb.y == -2) {
+        // norm in every l: rows of l-segment sg sit in the partials of its waves
+        // sg * wpl .. sg * wpl + wpl - 1 (wave table order), summed in that order
+        const int L = S.L;
+        const int nseg = S.nwaves / (S.npad / TDSE_WAVE);
+        const int seg = (L + nseg - 1) / nseg;
+        const int wpl = S.npad / TDSE_WAVE;
+        for (int l = t; l < L; l += blockDim.x) {
+            const int sg = l / seg, row = l - sg * seg;
+            double acc = 0.0;
+            for (int w = 0; w < wpl; ++w) acc += ld1(S.part_l, (size_t)(sg * wpl + w) * kShSegRows + row);
+            S.out_nl[(size_t)slot * L + l] = acc * S.delta_r;
+        }
+        return;
+    }
     if (jb.y < 0) {
         for (int w = t; w < S.nwaves; w += blockDim.x) {
             re += ld1(S.part, w);

@@ -126,6 +126,7 @@ class CompiledSH:
     g0: np.ndarray  # (L, npad) complex128, position order
     basis: np.ndarray  # (nb, npad) float64, position order
     basis_l: np.ndarray  # (nb,) int32
+    norm_by_l: bool = False  # also reduce the norm in every l at data times
 
     @property
     def npad(self):
@@ -193,7 +194,8 @@ def sh_radial_tables(*, r_bound, r_points, potential, mask, time_step, test_char
 
 def compile_sh(*, r_bound, r_points, l_bound, potential, field: Callable, mask, times, time_step,
                g0: np.ndarray, basis: Sequence[Tuple[np.ndarray, int]] = (), store_every=1,
-               test_charge=u.electron_charge, hydrogen_correction=True, rows=None) -> CompiledSH:
+               test_charge=u.electron_charge, hydrogen_correction=True, rows=None,
+               norm_by_l=False) -> CompiledSH:
     """
     Compile a length-gauge SphericalHarmonicMesh + SplitInteractionOperator
     simulation. ``times`` is the reference time grid (sims.py:198-205);
@@ -224,7 +226,7 @@ def compile_sh(*, r_bound, r_points, l_bound, potential, field: Callable, mask, 
         tables=np.ascontiguousarray(tables),
         fs=np.ascontiguousarray(fs), ts=np.ascontiguousarray(ts),
         g0=np.ascontiguousarray(to_positions(g0, P, m)),
-        basis=np.ascontiguousarray(to_positions(bmat, P, m)), basis_l=bl,
+        basis=np.ascontiguousarray(to_positions(bmat, P, m)), basis_l=bl, norm_by_l=bool(norm_by_l),
     )
 
 
@@ -460,6 +462,7 @@ class ShResult:
     norms: np.ndarray
     lnorms: np.ndarray
     ips: np.ndarray  # (n_data, nb)
+    norm_by_l: Optional[np.ndarray] = None  # (n_data, L) when the simulation reduces it
 
 
 class DeviceContext:
@@ -532,7 +535,7 @@ class DeviceContext:
     # -- simulations
     def add_sh(self, c: CompiledSH) -> int:
         desc = _lib.ShDesc(c.r_points, c.l_bound, c.P, c.m, c.n_steps, c.store_every, len(c.basis_l), c.lb,
-                           c.delta_r, c.corr0, c.centrifugal)
+                           c.delta_r, c.corr0, c.centrifugal, 1 if c.norm_by_l else 0, 0)
         slot = ctypes.c_int32()
         basis = np.ascontiguousarray(c.basis, dtype=np.float64) if len(c.basis_l) else None
         bl = np.ascontiguousarray(c.basis_l, dtype=np.int32)
@@ -617,7 +620,11 @@ class DeviceContext:
         self._check(self._lib.tdse_download(self._ctx, KIND_SH, slot, _lib.dptr(g), _lib.dptr(norms),
                                             _lib.dptr(lnorms), _lib.dptr(ips)))
         gn = from_positions(g, c.r_points, c.P, c.m) if with_g else None
-        return ShResult(gn, norms, lnorms, ips[:, : len(c.basis_l)])
+        nbl = None
+        if c.norm_by_l:
+            nbl = np.empty((nd, c.l_bound))
+            self._check(self._lib.tdse_download_norm_by_l(self._ctx, slot, _lib.dptr(nbl)))
+        return ShResult(gn, norms, lnorms, ips[:, : len(c.basis_l)], nbl)
 
     def download_line(self, slot: int):
         c = self._line[slot]

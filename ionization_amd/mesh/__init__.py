@@ -17,4 +17,4 @@ from .sims import (
     SphericalHarmonicSimulation,
     SphericalHarmonicSpecification,
 )
-from .data import Data, Datastore, Fields, Norm, InnerProducts, DEFAULT_DATASTORE_TYPES
+from .data import Data, Datastore, Fields, Norm, InnerProducts, NormBySphericalHarmonic, DEFAULT_DATASTORE_TYPES

@@ -163,6 +163,37 @@ DATA_NAME_TO_DATASTORE_TYPE.update({
     "total_state_overlap": InnerProducts,
 })
 
+class NormBySphericalHarmonic(Datastore):
+    """
+    Norm in every spherical harmonic vs time (data.py:438-464): sum over r of
+    |g_l(r)|^2 times delta_r (meshes.py:1130-1133), keyed by the spec's
+    ``spherical_harmonics``. On the device the L-phase of the step at a data
+    time reduces it per l while the state streams through, so it costs no
+    extra pass over the mesh.
+    """
+
+    def init(self, sim):
+        self.norm_by_l = {sph_harm: sim.get_blank_data() for sph_harm in sim.spec.spherical_harmonics}
+        super().init(sim)
+
+    def attach(self):
+        self.sim.data.norm_by_sph_harm = self.norm_by_l
+
+    def collect(self, slots, device):
+        by_l = device.get("norm_by_l")
+        if by_l is None:
+            raise exceptions.MissingDatastore(
+                f"the device did not reduce the norm by spherical harmonic for {self.sim}")
+        for sph_harm, col in zip(self.spec.spherical_harmonics, by_l.T):
+            self.norm_by_l[sph_harm][slots] = col[slots]
+
+    def store_host(self):
+        for sph_harm, l_norm in zip(self.spec.spherical_harmonics, self.sim.mesh.norm_by_l()):
+            self.norm_by_l[sph_harm][self.sim.data_time_index] = l_norm
+
+
+DATA_NAME_TO_DATASTORE_TYPE.update({"norm_by_sph_harm": NormBySphericalHarmonic})
+
 DEFAULT_DATASTORE_TYPES = (Fields, Norm, InnerProducts)
 
 

@@ -13,7 +13,7 @@ import numpy as np
 
 from .. import core, engine, potentials
 from .. import units as u
-from . import meshes
+from . import data, meshes
 
 
 class EvolutionMethod:
@@ -79,6 +79,7 @@ class SplitInteractionOperator(EvolutionMethod):
             test_charge=sim.spec.test_charge,
             hydrogen_correction=sim.spec.operators.hydrogen_zero_angular_momentum_correction,
             rows=getattr(sim.spec, "device_rows", None),
+            norm_by_l=data.NormBySphericalHarmonic in getattr(sim, "datastores_by_type", {}),
         )
 
 

@@ -23,7 +23,7 @@ import numpy as np
 
 from .. import engine, exceptions, potentials, states
 from .. import units as u
-from ..core import WarningRecord
+from ..core import SphericalHarmonic, WarningRecord
 from . import data, evolution_methods, mesh_operators, meshes
 
 logger = logging.getLogger(__name__)
@@ -66,7 +66,7 @@ class _DeviceSession:
     def data(self):
         if self.kind == engine.KIND_SH:
             res = self.ctx.download_sh(self.slot, with_g=False)
-            return {"norms": res.norms, "lnorms": res.lnorms, "ips": res.ips}
+            return {"norms": res.norms, "lnorms": res.lnorms, "ips": res.ips, "norm_by_l": res.norm_by_l}
         if self.kind == engine.KIND_CYL:
             _, norms, ips = self.ctx.download_cyl(self.slot, with_g=False)
             return {"norms": norms, "lnorms": None, "ips": ips}
@@ -399,4 +399,5 @@ class SphericalHarmonicSpecification(MeshSpecification):
         self.theta_points = theta_points
         self.use_numeric_eigenstates = use_numeric_eigenstates
         self.numeric_eigenstate_max_angular_momentum = min(self.l_bound - 1, numeric_eigenstate_max_angular_momentum)
+        self.spherical_harmonics = tuple(SphericalHarmonic(l, 0) for l in range(self.l_bound))
         self.numeric_eigenstate_max_energy = numeric_eigenstate_max_energy

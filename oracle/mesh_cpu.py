@@ -235,20 +235,28 @@ class SHOracle:
         """SphericalHarmonicMesh.inner_product shortcut (meshes.py:1117-1129)."""
         return np.sum(np.conj(radial_g) * g[l, :]) * self.delta_r
 
-    def run(self, g0, times, nominal_time_step, store_every=1, basis=()):
-        """MeshSimulation.run (sims.py:289-321): data first, then advance time_index and evolve."""
+    def norm_by_l(self, g):
+        """SphericalHarmonicMesh.norm_by_l (meshes.py:1130-1133): |sum_r conj(g) g| delta_r per l."""
+        return np.abs(np.sum(np.conj(g) * g, axis=1) * self.delta_r)
+
+    def run(self, g0, times, nominal_time_step, store_every=1, basis=(), by_l=False):
+        """MeshSimulation.run (sims.py:289-321): data first, then advance time_index and evolve.
+        by_l also returns the NormBySphericalHarmonic data (data.py:447-450), (n_data, L)."""
         g = np.array(g0, dtype=np.complex128)
         didx = set(data_indices(len(times), store_every).tolist())
-        norms, ips, lnorm = [], [], []
+        norms, ips, lnorm, nbl = [], [], [], []
         for i in range(len(times)):
             if i in didx:
                 norms.append(self.norm(g))
                 ips.append([self.inner_product(phi, l, g) for (phi, l) in basis])
                 lnorm.append((np.sum(np.conj(g[-1]) * g[-1]) * self.delta_r).real)
+                if by_l:
+                    nbl.append(self.norm_by_l(g))
             if i == len(times) - 1:
                 break
             g = self.evolve(g, times[i + 1] - times[i], times[i + 1], nominal_time_step)
-        return g, np.array(norms), np.array(ips, dtype=np.complex128), np.array(lnorm)
+        out = (g, np.array(norms), np.array(ips, dtype=np.complex128), np.array(lnorm))
+        return out + (np.array(nbl),) if by_l else out
 
 
 # ---------------------------------------------------------------------------

@@ -12,7 +12,8 @@ from oracle import mesh_cpu, physics
 
 
 def sh_case(*, r_bound_bohr=30.0, r_points=200, l_bound=20, pulse_width_as=100.0, fluence_jcm2=1.0,
-            phase=0.0, t_half_as=300.0, dt_as=1.0, n_basis_l=3, store_every=1, mask=True, rows=None):
+            phase=0.0, t_half_as=300.0, dt_as=1.0, n_basis_l=3, store_every=1, mask=True, rows=None,
+            norm_by_l=False):
     """Length-gauge SH + split operator with a sinc pulse, an absorber and a numeric 1s start."""
     r_bound = r_bound_bohr * u.bohr_radius
     pw = pulse_width_as * u.asec
@@ -37,7 +38,7 @@ def sh_case(*, r_bound_bohr=30.0, r_points=200, l_bound=20, pulse_width_as=100.0
     times = mesh_cpu.time_grid(-t_half_as * u.asec, t_half_as * u.asec, dt)
     comp = engine.compile_sh(r_bound=r_bound, r_points=r_points, l_bound=l_bound, potential=physics.coulomb,
                              field=field, mask=maskf, times=times, time_step=dt, g0=g0, basis=basis,
-                             store_every=store_every, rows=rows)
+                             store_every=store_every, rows=rows, norm_by_l=norm_by_l)
     return dict(oracle=orc, compiled=comp, g0=g0, times=times, dt=dt, basis=basis, store_every=store_every)
 
 

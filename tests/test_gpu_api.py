@@ -23,7 +23,7 @@ def _spec(name="api", **kw):
     return mesh.SphericalHarmonicSpecification(name, **args)
 
 
-def _oracle_run(sim):
+def _oracle_run(sim, by_l=False):
     spec = sim.spec
     pulse = spec.electric_potential
     field = lambda t: physics.sinc_pulse_field(t, pulse_width=pulse.pulse_width, fluence=pulse.fluence,
@@ -32,7 +32,7 @@ def _oracle_run(sim):
     orc = mesh_cpu.SHOracle(r_bound=rb, r_points=spec.r_points, l_bound=spec.l_bound, potential=physics.coulomb,
                             field=field, mask=lambda r: physics.radial_cosine_mask(r, 25 * u.bohr_radius, rb))
     basis = [(np.real(sim.mesh.get_radial_g_for_state(s)), s.l) for s in spec.test_states]
-    return orc.run(sim.mesh.g, sim.times, spec.time_step, spec.store_data_every, basis=basis)
+    return orc.run(sim.mesh.g, sim.times, spec.time_step, spec.store_data_every, basis=basis, by_l=by_l)
 
 
 def test_spec_run_matches_oracle():
@@ -50,6 +50,20 @@ def test_spec_run_matches_oracle():
     assert np.max(np.abs((1 - sim.data.bound_state_overlap) - (1 - ref_bound))) <= 1e-9
     assert np.allclose(sim.lnorm, lnorms, atol=1e-9)
     assert not np.array_equal(g0, sim.mesh.g)
+
+
+def test_norm_by_spherical_harmonic_datastore_matches_oracle():
+    """data.py:438-458: sim.data.norm_by_sph_harm keyed by the spec's spherical harmonics."""
+    stores = [mesh.Fields(), mesh.Norm(), mesh.InnerProducts(), mesh.NormBySphericalHarmonic()]
+    sim = _spec("by_l", l_bound=70, store_data_every=6, datastores=stores).to_sim()
+    *_, nbl = _oracle_run(sim, by_l=True)
+    sim.run()
+    harmonics = sim.spec.spherical_harmonics
+    assert [h.l for h in harmonics] == list(range(70)) and all(h.m == 0 for h in harmonics)
+    dev = np.array([sim.data.norm_by_sph_harm[h] for h in harmonics]).T
+    assert dev.shape == nbl.shape
+    assert np.max(np.abs(dev - nbl)) <= 1e-9
+    assert np.max(np.abs(dev.sum(axis=1) - sim.data.norm)) <= 1e-13
 
 
 def test_callback_mode_matches_batch_mode():

@@ -84,6 +84,45 @@ def test_sh_split_operator_matches_oracle(device_ctx, r_points, l_bound):
     assert abs((1 - bound_dev) - (1 - bound_orc)) <= OBS_ATOL
 
 
+@pytest.mark.parametrize(
+    "r_points, l_bound, store_every",
+    # one l-segment (even and odd L) and several segments of uneven length
+    [(200, 20, 5), (201, 21, 3), (96, 150, 4), (70, 257, 6)],
+)
+def test_sh_norm_by_l_matches_oracle(device_ctx, r_points, l_bound, store_every):
+    """NormBySphericalHarmonic reduced in the L-phase against meshes.py:1130-1133 at every data time."""
+    case = sh_case(r_points=r_points, l_bound=l_bound, t_half_as=60.0, store_every=store_every, norm_by_l=True)
+    comp = case["compiled"]
+    print(f"SH norm by l r_points={r_points} l_bound={l_bound} every={store_every}", flush=True)
+    assert comp.norm_by_l
+    res = _run_sh_device(device_ctx, comp)
+    g, norms, ips, lnorm, nbl = case["oracle"].run(case["g0"], case["times"], case["dt"], store_every,
+                                                  basis=case["basis"], by_l=True)
+    assert res.norm_by_l.shape == nbl.shape == (len(norms), l_bound)
+    assert np.max(np.abs(res.norm_by_l - nbl)) <= OBS_ATOL
+    assert np.max(np.abs(res.norm_by_l.sum(axis=1) - res.norms)) <= 1e-13
+    assert np.max(np.abs(res.norm_by_l[:, -1] - res.lnorms)) <= 1e-13
+    # the wavefunction and the other observables are those of the run without it
+    assert rel_l2(res.g, g) <= WF_RTOL
+    assert np.max(np.abs(res.norms - norms)) <= OBS_ATOL
+    assert np.max(np.abs(res.ips - ips)) <= OBS_ATOL
+
+
+def test_sh_norm_by_l_does_not_change_other_results(device_ctx):
+    """The same simulation with and without the per-l reduction, side by side in one batch."""
+    plain = sh_case(r_points=150, l_bound=70, t_half_as=40.0, store_every=3)["compiled"]
+    by_l = sh_case(r_points=150, l_bound=70, t_half_as=40.0, store_every=3, norm_by_l=True)["compiled"]
+    s0, s1 = device_ctx.add_sh(plain), device_ctx.add_sh(by_l)
+    device_ctx.run(plain.n_steps + 1)
+    a, b = device_ctx.download_sh(s0), device_ctx.download_sh(s1)
+    device_ctx.remove(0, s0)
+    device_ctx.remove(0, s1)
+    assert a.norm_by_l is None
+    assert np.array_equal(a.g, b.g)
+    assert np.array_equal(a.norms, b.norms)
+    assert np.array_equal(a.ips, b.ips)
+
+
 @pytest.mark.parametrize("rows", [2, 4, 6, 8, 10, 12, 14, 16])
 def test_sh_every_rows_per_thread_matches_oracle(device_ctx, rows):
     """Every R-phase instance (rows per thread m, block size, explicit or Cayley form) on a

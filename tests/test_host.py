@@ -166,6 +166,19 @@ def test_data_access_errors():
         sim.data.definitely_not_data
     with pytest.raises(exceptions.DuplicateDatastores):
         _small_sh_spec(datastores=[mesh.Norm(), mesh.Norm()])
+    with pytest.raises(exceptions.MissingDatastore):
+        sim.data.norm_by_sph_harm
+
+
+def test_norm_by_spherical_harmonic_compiles_the_device_reduction():
+    plain = _small_sh_spec().to_sim()
+    assert not plain.spec.evolution_method.compile(plain).norm_by_l
+    sim = _small_sh_spec(datastores=[mesh.Norm(), mesh.NormBySphericalHarmonic()]).to_sim()
+    assert sim.spec.evolution_method.compile(sim).norm_by_l
+    harmonics = sim.spec.spherical_harmonics
+    assert len(harmonics) == sim.spec.l_bound and len(set(harmonics)) == len(harmonics)
+    assert set(sim.data.norm_by_sph_harm) == set(harmonics)
+    assert all(np.isnan(v).all() and len(v) == sim.data_time_steps for v in sim.data.norm_by_sph_harm.values())
 
 
 def test_scan_draws_are_seeded_and_in_range():

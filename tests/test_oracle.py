@@ -193,6 +193,36 @@ def test_sh_evolution_without_mask_is_unitary():
     assert abs(norms[-1] - norms[0]) < 1e-12 * norms[0]
 
 
+def test_sh_norm_by_l_pinned_by_total_norm_and_pure_l_states():
+    """
+    NormBySphericalHarmonic (meshes.py:1130-1133): the per-l norms sum to the
+    norm, the entry at l = L-1 is the largest-l norm the run stores separately
+    (sims.py:245-256), and a state in one l keeps its whole norm there while no
+    field couples it to its neighbours.
+    """
+    fld = lambda t: physics.sinc_pulse_field(t, pulse_width=50 * u.asec, fluence=1 * u.Jcm2)
+    o = mesh_cpu.SHOracle(r_bound=20 * u.bohr_radius, r_points=60, l_bound=8, potential=physics.coulomb, field=fld)
+    rng = np.random.default_rng(9)
+    g = rng.standard_normal((8, 60)) + 1j * rng.standard_normal((8, 60))
+    times = mesh_cpu.time_grid(-30 * u.asec, 30 * u.asec, 1 * u.asec)
+    _, norms, _, lnorm, nbl = o.run(g, times, 1 * u.asec, store_every=7, by_l=True)
+    assert nbl.shape == (len(norms), 8)
+    assert np.all(nbl >= 0)
+    assert np.allclose(nbl.sum(axis=1), norms, rtol=1e-13, atol=0)
+    assert np.allclose(nbl[:, -1], lnorm, rtol=1e-13, atol=0)
+    # the first data point is the input itself, row by row
+    ref0 = np.array([np.vdot(g[l], g[l]).real for l in range(8)]) * o.delta_r
+    assert np.allclose(nbl[0], ref0, rtol=1e-13, atol=0)
+
+    free = mesh_cpu.SHOracle(r_bound=20 * u.bohr_radius, r_points=60, l_bound=8, potential=physics.coulomb,
+                             field=lambda t: 0.0 * t)
+    g1 = np.zeros((8, 60), dtype=np.complex128)
+    g1[3] = rng.standard_normal(60)
+    _, n1, _, _, nbl1 = free.run(g1, times, 1 * u.asec, store_every=10, by_l=True)
+    assert np.allclose(nbl1[:, 3], n1, rtol=1e-13, atol=0)
+    assert np.all(np.delete(nbl1, 3, axis=1) == 0.0)
+
+
 # ------------------------------------------------------------------ Line and Cyl
 
 
