@@ -436,6 +436,8 @@ def main():
     served = (time_serve(args.serve_requests, args.serve_rate, args.seed, args.serve_pool_gb)
               if args.serve_requests > 0 else None)
 
+    copy_gbs = calibrate_copy()  # attainable streaming rate, beside the published peak
+
     cpu = None
     if args.cpu_seconds > 0:
         c = cpu_baseline(params, args.cpu_seconds)
@@ -478,6 +480,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "copy_gbs": copy_gbs,
         },
         "cpu_baseline": cpu,
         "cyl_adi": cyl,

@@ -245,8 +245,8 @@ __global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, con
         if (do_start) sh_rotate(f_start * c_m3, o_a, o_b);
         const int lo_row = v0 - 3 - q;
         if (live) {
-            if (lo_row >= lo && lo_row < hi) st2(g, (size_t)lo_row * npad, o_a);
-            if (lo_row + 1 >= lo && lo_row + 1 < hi) st2(g, (size_t)(lo_row + 1) * npad, o_b);
+            if (lo_row >= lo && lo_row < hi) st2s(g, (size_t)lo_row * npad, o_a);
+            if (lo_row + 1 >= lo && lo_row + 1 < hi) st2s(g, (size_t)(lo_row + 1) * npad, o_b);
         }
         carry_u = u1;
         carry_s1 = s10;
@@ -349,7 +349,7 @@ __device__ __forceinline__ void cn_line_finish(double2 (&x)[M], double2* __restr
         chunk_neighbours<M>(x, k, P, lds, xl, xr, false);
         cn_implicit_lowreg<M, Coef, Coef2, true>(x, k, P, coef, coef2, lds, xl, xr);
 #pragma unroll
-        for (int i = 0; i < M; ++i) st2(gl, (size_t)i * P, x[i]);
+        for (int i = 0; i < M; ++i) st2s(gl, (size_t)i * P, x[i]);
     } else {
         double2* keep = lds + 3 * P;  // CN_CAYLEY_LDS: this thread's own rows only
         if constexpr (FORM == CN_CAYLEY_LDS) {
@@ -360,7 +360,7 @@ __device__ __forceinline__ void cn_line_finish(double2 (&x)[M], double2* __restr
 #pragma unroll
         for (int i = 0; i < M; ++i) {
             const double2 x0 = (FORM == CN_CAYLEY_LDS) ? keep[i * P + k] : ld2(gl, (size_t)i * P);
-            st2(gl, (size_t)i * P, cmk(fma(2.0, x[i].x, -x0.x), fma(2.0, x[i].y, -x0.y)));
+            st2s(gl, (size_t)i * P, cmk(fma(2.0, x[i].x, -x0.x), fma(2.0, x[i].y, -x0.y)));
         }
     }
 }
@@ -385,7 +385,8 @@ template <int M, int MAXT, int WPE, int FORM>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_sh_rphase(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines) {
     extern __shared__ double2 lds[];  // 3 * P, then M * P for CN_CAYLEY_LDS
-    const int line = blockIdx.x;
+    // consecutive l-lines of a simulation (which share its coefficient rows) on one XCD
+    const int line = xcd_remap(blockIdx.x, gridDim.x);
     if (line >= n_lines) return;
     const int2 lt = lines[line];
     ShSim& S = sims[lt.x];
@@ -408,7 +409,7 @@ k_sh_rphase(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_line
 
     double2 x[M];
 #pragma unroll
-    for (int i = 0; i < M; ++i) x[i] = ld2(gl, (size_t)i * P);
+    for (int i = 0; i < M; ++i) x[i] = (FORM == CN_CAYLEY_MEM) ? ld2(gl, (size_t)i * P) : ld2s(gl, (size_t)i * P);
     auto diag = [&](int i, double T) {
         const int j = k * M + i;
         const double r = fma((double)j, dr, 0.5 * dr);
