@@ -40,9 +40,11 @@ def choose_layout(n: int, kind: str = "sh", rows: Optional[int] = None) -> Tuple
     """
     Pick (P, m) for a line of n points: P a multiple of 64, m even, P m >= n,
     within the kernel limits (m <= 8 -> P <= 512, m <= 16 -> P <= 256; Line:
-    P <= 512). SH radial lines: least padding, then the most rows per thread
-    (fewer, longer chunks: a shorter reduced system and more work per thread;
-    measured fastest on the 256-simulation scan). Line and cylindrical
+    P <= 512). SH radial lines: P a power of two where one fits (workgroups of
+    1, 2 or 4 waves per SIMD set; on the 256-simulation scan 3-wave workgroups
+    of P = 192 made the R-phase slower: 1.65 -> 1.59 ms/step without them),
+    then least padding, then the most rows per thread (fewer, longer chunks:
+    a shorter reduced system and more work per thread). Line and cylindrical
     z-lines: least padding, then m = 8, then the most rows.
     """
     limits = {"sh": {8: 512, 16: 256}, "cylz": {8: 512, 16: 256}, "line": {8: 512, 16: 512}}[kind]
@@ -53,6 +55,12 @@ def choose_layout(n: int, kind: str = "sh", rows: Optional[int] = None) -> Tuple
             continue
         cap = limits[8] if m <= 8 else limits[16]
         P = 64 * -(-n // (64 * m))
+        if kind == "sh":
+            p2 = 64
+            while p2 < P:
+                p2 *= 2
+            if p2 <= cap:
+                P = p2
         if P > cap:
             continue
         key = (P * m - n, -m) if kind == "sh" else (P * m - n, 0 if m == 8 else 1, -m)

@@ -126,7 +126,7 @@ def test_sh_norm_by_l_does_not_change_other_results(device_ctx):
 @pytest.mark.parametrize("rows", [2, 4, 6, 8, 10, 12, 14, 16])
 def test_sh_every_rows_per_thread_matches_oracle(device_ctx, rows):
     """Every R-phase instance (rows per thread m, block size, explicit or Cayley form) on a
-    700-point radial line: m = 2..16 gives P = 384, 192, 128, 128, 128, 64, 64, 64."""
+    700-point radial line: m = 2..16 gives P = 512, 256, 128, 128, 128, 64, 64, 64."""
     case = sh_case(r_points=700, l_bound=6, t_half_as=20.0, rows=rows)
     comp = case["compiled"]
     print(f"SH rows={rows} P={comp.P} m={comp.m} steps={comp.n_steps}", flush=True)
