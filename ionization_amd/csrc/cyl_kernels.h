@@ -50,6 +50,7 @@ struct CylSim {
     double m_inner, m_outer, m_inv_s;  // RadialCosineMask (inner, outer, 1 / smoothness)
     double z0, dz, rho0, drho;         // grids: z_j = z0 + j dz, rho_j = rho0 + j drho
     int has_mask;
+    int m_roots;  // smoothness = 2^m_roots: the root is m_roots square roots; -1 otherwise
     int nz, nr, Pz, Mz, npz, Cr, Mr, nrpad, lanes, n_groups;
     int N, store_every, n_basis;
     int step;   // steps already advanced; device-updated
@@ -89,6 +90,13 @@ __device__ __forceinline__ double cyl_mask_r(const CylSim& S, double r) {
     if (r >= S.m_outer) return 0.0;
     double sn, cs;
     tdse_sincos(0.5 * 3.141592653589793 * (r - S.m_inner) / fabs(S.m_outer - S.m_inner), &sn, &cs);
+    // |cos|^(1/s): for the usual s = 2^k (default 8) k square roots, a few
+    // times cheaper than pow (1.49 -> 1.45 ms per cylinder step)
+    if (S.m_roots >= 0) {
+        double c = fabs(cs);
+        for (int i = 0; i < S.m_roots; ++i) c = sqrt(c);
+        return c;
+    }
     return pow(fabs(cs), S.m_inv_s);
 }
 

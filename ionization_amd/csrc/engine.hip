@@ -979,6 +979,9 @@ int tdse_cyl_add(tdse_ctx* ctx, const tdse_cyl_desc* desc, const double* zt, con
     S.m_inner = D.mask_inner;
     S.m_outer = D.mask_outer;
     S.m_inv_s = D.mask_smoothness > 0 ? 1.0 / D.mask_smoothness : 1.0;
+    S.m_roots = -1;
+    for (int k = 0; k <= 10; ++k)
+        if (D.mask_smoothness == (double)(1 << k)) S.m_roots = k;
     S.z0 = D.z_first;
     S.dz = D.z_step;
     S.rho0 = D.rho_first;

@@ -70,15 +70,16 @@ def line_case(*, z_bound_bohr=50.0, z_points=500, n_steps=300, e0_au=0.05, omega
 
 
 def cyl_case(*, z_bound_bohr=20.0, rho_bound_bohr=10.0, z_points=120, rho_points=60, pulse_width_as=50.0,
-             fluence_jcm2=5.0, t_half_as=40.0, dt_as=1.0, store_every=1, mask=True, z_rows=None, rho_shape=None):
+             fluence_jcm2=5.0, t_half_as=40.0, dt_as=1.0, store_every=1, mask=True, z_rows=None, rho_shape=None,
+             mask_smoothness=8.0):
     """CylindricalSliceMesh ADI with a sinc pulse, a radial absorber and a hydrogen 1s start."""
     zb = z_bound_bohr * u.bohr_radius
     rb = rho_bound_bohr * u.bohr_radius
     pw = pulse_width_as * u.asec
     F = fluence_jcm2 * u.Jcm2
     field = lambda t: physics.sinc_pulse_field(t, pulse_width=pw, fluence=F, phase=0.0)
-    mask_params = (0.7 * zb, zb, 8.0) if mask else None
-    maskf = (lambda r: physics.radial_cosine_mask(r, mask_params[0], mask_params[1])) if mask else None
+    mask_params = (0.7 * zb, zb, mask_smoothness) if mask else None
+    maskf = (lambda r: physics.radial_cosine_mask(r, mask_params[0], mask_params[1], mask_smoothness)) if mask else None
     orc = mesh_cpu.CylOracle(z_bound=zb, rho_bound=rb, z_points=z_points, rho_points=rho_points,
                              potential=physics.coulomb, field=field, mask=maskf)
     w = orc.delta_z * orc.delta_rho

@@ -50,6 +50,14 @@ def test_cyl_adi_matches_oracle_every_rho_shape(device_ctx, rho_shape):
     _check(case, *_run_device(device_ctx, c))
 
 
+@pytest.mark.parametrize("smoothness", [1.0, 2.0, 3.0, 8.0, 16.0, 5.5])
+def test_cyl_mask_smoothness(device_ctx, smoothness):
+    """|cos|^(1/s): square roots for s = 2^k, pow otherwise (potentials/masks.py:76-89)."""
+    case = cyl_case(z_points=96, rho_points=40, t_half_as=30.0, store_every=3, mask_smoothness=smoothness)
+    print(f"Cyl mask smoothness={smoothness}", flush=True)
+    _check(case, *_run_device(device_ctx, case["compiled"]))
+
+
 @pytest.mark.parametrize("z_rows", [2, 8, 10, 16])
 def test_cyl_adi_matches_oracle_z_rows(device_ctx, z_rows):
     case = cyl_case(z_points=150, rho_points=44, z_rows=z_rows, t_half_as=15.0, store_every=4)
