@@ -5,7 +5,7 @@
 //    |r| <= pi/4, where the first omitted terms are below 1e-19. Valid for
 //    |x| < 2^20 pi/2; the angles of the split-operator rotations are a few
 //    radians at most.
-//  * tdse_rcp: hardware reciprocal estimate refined by two Newton steps.
+//  * tdse_rcp: hardware reciprocal estimate refined to third order.
 // The header also compiles for the host (without HIP) so the same code can
 // be checked against long-double libm on the CPU.
 #pragma once
@@ -64,10 +64,13 @@ TDSE_HD void tdse_sincos(double x, double* s, double* c) {
 
 TDSE_HD double tdse_rcp(double d) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    double r = __builtin_amdgcn_rcp(d);
-    r = std::fma(std::fma(-d, r, 1.0), r, r);
-    r = std::fma(std::fma(-d, r, 1.0), r, r);
-    return r;
+    // 1/d = r0 / (1 - e) = r0 (1 + e + e^2 + ...), e = 1 - d r0: the estimate
+    // (relative error below 2^-24 on gfx950) refined by the series to third
+    // order, error ~e^3 below the final rounding; one FMA fewer than two
+    // Newton steps
+    const double r = __builtin_amdgcn_rcp(d);
+    const double e = std::fma(-d, r, 1.0);
+    return std::fma(r, std::fma(e, e, e), r);
 #else
     return 1.0 / d;
 #endif
