@@ -354,28 +354,16 @@ void launch_rphase(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
                            st, ctx->d_sh, b.lines.ptr, (int)b.lines.count);
 }
 
-#ifndef RW10
-#define RW10 3
-#endif
-#ifndef RW12
-#define RW12 2
-#endif
-#ifndef RW14
-#define RW14 2
-#endif
-#ifndef RW16
-#define RW16 1
-#endif
 void dispatch_rphase(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
     switch (b.M) {
         case 2: launch_rphase<2, 512, 4>(ctx, b, st); break;
         case 4: launch_rphase<4, 512, 4>(ctx, b, st); break;
         case 6: launch_rphase<6, 512, 4>(ctx, b, st); break;
         case 8: launch_rphase<8, 512, 4>(ctx, b, st); break;
-        case 10: launch_rphase<10, 256, RW10>(ctx, b, st); break;
-        case 12: launch_rphase<12, 256, RW12>(ctx, b, st); break;
-        case 14: launch_rphase<14, 256, RW14>(ctx, b, st); break;
-        default: launch_rphase<16, 256, RW16>(ctx, b, st); break;
+        case 10: launch_rphase<10, 256, 3>(ctx, b, st); break;
+        case 12: launch_rphase<12, 256, 2>(ctx, b, st); break;
+        case 14: launch_rphase<14, 256, 2>(ctx, b, st); break;
+        default: launch_rphase<16, 256, 1>(ctx, b, st); break;
     }
 }
 
