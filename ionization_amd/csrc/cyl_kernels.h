@@ -141,7 +141,14 @@ k_cyl_z(CylSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines) 
         double hp;
         coef(i, hm, h, hp);
     };
-    cn_line_finish<M, FORM>(x, gl, k, P, coef, coef2, lds);
+    // equal off-diagonals inside the line: hm / hp = 1 from row 1 to nz - 2, so
+    // pass 2 reuses the stored pivots; row nz - 1 (hp = 0) is rebuilt
+    auto ratio = [&](int i) {
+        const int j = k * M + i;
+        return (j >= 1 && j + 1 < nz) ? 1.0 : 0.0;
+    };
+    const int ib = ((nz - 1) / M == k) ? (nz - 1) % M : -1;
+    cn_line_finish<M, FORM>(x, gl, k, P, coef, coef2, lds, ratio, ib);
 }
 
 // ---------------------------------------------------------------- rho-sweep

@@ -142,7 +142,15 @@ __global__ void __launch_bounds__(MAXT) k_line_steps(LineSim* __restrict__ sims,
                 h = sw * fma(f, zq, ld1(H2, i * P));
                 hm = (j >= 1 && j < nz) ? so : 0.0;
             };
-            cn_implicit_lowreg<M, decltype(coef), decltype(coef2), true>(x, k, P, coef, coef2, lds, xl, xr);
+            // equal off-diagonals inside the line: pass 2 reuses the stored pivots
+            // (hm / hp = 1 from row 1 to nz - 2); row nz - 1 (hp = 0) is rebuilt
+            auto ratio = [&](int i) {
+                const int j = k * M + i;
+                return (j >= 1 && j + 1 < nz) ? 1.0 : 0.0;
+            };
+            const int ib = ((nz - 1) / M == k) ? (nz - 1) % M : -1;
+            cn_implicit_lowreg<M, decltype(coef), decltype(coef2), true, 1, decltype(ratio)>(
+                x, k, P, coef, coef2, lds, xl, xr, 0, ratio, ib);
         } else {
             auto coef = [&](int i, double& hm, double& h, double& hp) {
                 h = fma(f, ld1(ZQ, i * P), ld1(H, i * P));

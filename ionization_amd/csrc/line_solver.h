@@ -25,6 +25,8 @@
 // physical line are padding rows with H = 0 and x = 0 (they stay 0).
 #pragma once
 
+#include <type_traits>
+
 #include "common.h"
 
 template <bool IMAG>
@@ -286,11 +288,20 @@ __device__ __forceinline__ void cn_cayley_solve(double2 (&x)[M], const int k, co
 // row by row in pass 1 from x and the chunk neighbours xl = x_{km-1},
 // xr = x_{km+m} (chunk_neighbours); otherwise it is x itself.
 // LANES / ln: several interleaved lines per block, as for reduced_cr.
-template <int M, class Coef, class Coef2, bool EXPLICIT = false, int LANES = 1>
+//
+// Ratio: when given (not NoRatio), pass 2 takes the sub-diagonal pivot factor
+// rsub_i = i hm_i / den_i from the stored super-diagonal one, g_i = i hp_i / den_i,
+// as rsub_i = g_i (hm_i / hp_i), with ratio(i) = hm_i / hp_i (0 where hm_i = 0):
+// a product instead of rebuilding den_i and its reciprocal. Row ib of the thread
+// (the one row of the line with hp = 0 and hm != 0, or -1) is rebuilt as before.
+struct NoRatio {};
+
+template <int M, class Coef, class Coef2, bool EXPLICIT = false, int LANES = 1, class Ratio = NoRatio>
 __device__ __forceinline__ void cn_implicit_lowreg(double2 (&x)[M], const int k, const int P, Coef coef,
                                                    Coef2 coef2, double2* __restrict__ lds,
                                                    const double2 xl = make_double2(0.0, 0.0),
-                                                   const double2 xr = make_double2(0.0, 0.0), const int ln = 0) {
+                                                   const double2 xr = make_double2(0.0, 0.0), const int ln = 0,
+                                                   Ratio ratio = Ratio{}, const int ib = -1) {
     double2 ga[M - 1], de[M - 1];
     double2 al = cmk(0.0, 0.0), pi = cmk(1.0, 0.0), sa = cmk(0.0, 0.0), sd = cmk(0.0, 0.0);
     double2 gprev = cmk(0.0, 0.0), dprev = cmk(0.0, 0.0);
@@ -360,11 +371,21 @@ __device__ __forceinline__ void cn_implicit_lowreg(double2 (&x)[M], const int k,
     double2 mu = cmk(0.0, 0.0);
 #pragma unroll
     for (int i = 0; i < M - 1; ++i) {
-        double hm, h;
-        coef2(i, hm, h);
-        const double2 den = (i == 0) ? cmk(1.0, h) : cmk(fma(hm, ga[i - 1 > 0 ? i - 1 : 0].y, 1.0),
-                                                          fma(-hm, ga[i - 1 > 0 ? i - 1 : 0].x, h));
-        const double2 rsub = cimul(hm, crecip(den));
+        double2 rsub;
+        auto direct = [&]() {
+            double hm, h;
+            coef2(i, hm, h);
+            const double2 den = (i == 0) ? cmk(1.0, h) : cmk(fma(hm, ga[i - 1 > 0 ? i - 1 : 0].y, 1.0),
+                                                              fma(-hm, ga[i - 1 > 0 ? i - 1 : 0].x, h));
+            return cimul(hm, crecip(den));
+        };
+        if constexpr (std::is_same_v<Ratio, NoRatio>) {
+            rsub = direct();
+        } else {
+            const double q = ratio(i);
+            rsub = cmk(ga[i].x * q, ga[i].y * q);
+            if (i == ib) rsub = direct();
+        }
         mu = (i == 0) ? cmul(rsub, Lk) : cneg(cmul(rsub, mu));
         de[i] = csub(de[i], mu);
     }

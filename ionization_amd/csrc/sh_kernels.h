@@ -341,13 +341,15 @@ enum { CN_CAYLEY_MEM = 0, CN_CAYLEY_LDS = 1, CN_EXPLICIT = 2 };
 
 // Shared tail of the CN line kernels: x holds the thread's M rows of the line
 // at gl (stride P); solves and stores y = (1 + i H~)^-1 (1 - i H~) x.
-template <int M, int FORM, class Coef, class Coef2>
+// Ratio / ib: optional pass-2 pivot ratios (cn_implicit_lowreg).
+template <int M, int FORM, class Coef, class Coef2, class Ratio = NoRatio>
 __device__ __forceinline__ void cn_line_finish(double2 (&x)[M], double2* __restrict__ gl, const int k, const int P,
-                                               Coef coef, Coef2 coef2, double2* __restrict__ lds) {
+                                               Coef coef, Coef2 coef2, double2* __restrict__ lds,
+                                               Ratio ratio = Ratio{}, const int ib = -1) {
     if constexpr (FORM == CN_EXPLICIT) {
         double2 xl, xr;
         chunk_neighbours<M>(x, k, P, lds, xl, xr, false);
-        cn_implicit_lowreg<M, Coef, Coef2, true>(x, k, P, coef, coef2, lds, xl, xr);
+        cn_implicit_lowreg<M, Coef, Coef2, true, 1, Ratio>(x, k, P, coef, coef2, lds, xl, xr, 0, ratio, ib);
 #pragma unroll
         for (int i = 0; i < M; ++i) st2s(gl, (size_t)i * P, x[i]);
     } else {
@@ -356,7 +358,8 @@ __device__ __forceinline__ void cn_line_finish(double2 (&x)[M], double2* __restr
 #pragma unroll
             for (int i = 0; i < M; ++i) keep[i * P + k] = x[i];
         }
-        cn_implicit_lowreg<M>(x, k, P, coef, coef2, lds);
+        cn_implicit_lowreg<M, Coef, Coef2, false, 1, Ratio>(x, k, P, coef, coef2, lds, cmk(0.0, 0.0), cmk(0.0, 0.0),
+                                                            0, ratio, ib);
 #pragma unroll
         for (int i = 0; i < M; ++i) {
             const double2 x0 = (FORM == CN_CAYLEY_LDS) ? keep[i * P + k] : ld2(gl, (size_t)i * P);
