@@ -246,14 +246,25 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 
 
     if constexpr (MODE == CYL_MAIN) {
         const double tau = ld1(S.ts, n), E = ld1(S.fs, n);
-        double carry1 = a_left, carry2 = a_left;
+        double carry1 = a_left, carry3 = a_left;
         auto coef = [&](int i, double& hm, double& h, double& hp) { rho_coef(carry1, tau, E, i, hm, h, hp); };
+        // row i's coefficients on their own (the one row whose pass-2 pivot is rebuilt)
         auto coef2 = [&](int i, double& hm, double& h) {
-            double hp;
-            rho_coef(carry2, tau, E, i, hm, h, hp);
+            double hp, left = cyl_rho_off(S, c * M + i - 1);
+            rho_coef(left, tau, E, i, hm, h, hp);
         };
-        cn_implicit_lowreg<M, decltype(coef), decltype(coef2), false, LANES>(x, c, P, coef, coef2, lds,
-                                                                            cmk(0.0, 0.0), cmk(0.0, 0.0), ln);
+        // pass-2 pivot ratios hm / hp = c(j) / c(j + 1) (tau cancels), rows in order
+        auto ratio = [&](int i) {
+            const int j = c * M + i;
+            const bool live = line_live && j < nr;
+            const double off = live ? cyl_rho_off(S, j) : 0.0;
+            const double q = (live && j >= 1 && off != 0.0) ? carry3 * tdse_rcp(off) : 0.0;
+            carry3 = off;
+            return q;
+        };
+        const int ib = (line_live && (nr - 1) / M == c) ? (nr - 1) % M : -1;
+        cn_implicit_lowreg<M, decltype(coef), decltype(coef2), false, LANES, decltype(ratio)>(
+            x, c, P, coef, coef2, lds, cmk(0.0, 0.0), cmk(0.0, 0.0), ln, ratio, ib);
         if (S.has_mask) {
 #pragma unroll
             for (int i = 0; i < M; ++i) {
