@@ -143,9 +143,13 @@ __global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, con
     const double f_sum = f_end + f_start;
     const double2* __restrict__ gin = sh_state(S, n) + p;
     double2* __restrict__ g = sh_state(S, n + 1) + p;
-    auto load_row = [gin, live, L, npad](int l) -> double2 {
+    // Output row r depends on input rows up to r + 3, so rows from hi + 3 on
+    // (which the prefetch ring would otherwise read up to 2 PF rows ahead) are
+    // not loaded: the next segment's wave reads them.
+    const int l_stop = min(L, hi + 3);
+    auto load_row = [gin, live, l_stop, npad](int l) -> double2 {
         double2 v = make_double2(0.0, 0.0);
-        if (live && l >= 0 && l < L) v = ld2(gin, (size_t)l * npad);
+        if (live && l >= 0 && l < l_stop) v = ld2(gin, (size_t)l * npad);
         return v;
     };
     // coupling of the pair starting at virtual row v (0 when the pair does not exist:
