@@ -381,6 +381,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="also measure every candidate SH layout per radial size class and time the scan "
+                         "under the measured layouts (reported as 'autotuned')")
     ap.add_argument("--cyl-sims", type=int, default=32, help="0 skips the cylindrical ADI workload")
     ap.add_argument("--cyl-steps", type=int, default=100)
     ap.add_argument("--line-batch", type=int, default=256, help="0 skips the LineMesh workload")
@@ -413,6 +416,25 @@ def main():
         pl = time_plain(args.plain_steps, 10)
         plain_rate = pl["points"] * args.plain_steps / pl["wall_s"]
         log(f"plain: {pl['wall_s'] * 1e3 / args.plain_steps:.3f} ms/step, {plain_rate / 1e9:.3f} G mesh-point-steps/s")
+
+    tuned = None
+    if args.autotune:
+        from ionization_amd import autotune, engine
+
+        t = time.perf_counter()
+        table, trial = autotune.tune_batch(compiled, log=log)
+        tune_s = time.perf_counter() - t
+        engine.set_sh_layout_table(table)
+        relaid = [engine.relayout_sh(c) for c in compiled]
+        engine.set_sh_layout_table({})
+        bt = time_batched(relaid, args.steps, args.warmup)
+        rate_t = bt["points"] * args.steps / bt["wall_s"]
+        tuned = {"value": rate_t, "ms_per_step": bt["wall_s"] * 1e3 / args.steps, "tune_s": tune_s,
+                 "size_classes": len(table), "trial_ms_per_step": trial,
+                 "layouts_changed": sum((c.P, c.m) != (d.P, d.m) for c, d in zip(compiled, relaid)),
+                 "table": {str(k): list(v) for k, v in sorted(table.items())}}
+        log(f"autotuned: {tuned['ms_per_step']:.3f} ms/step, {rate_t / 1e9:.2f} G mesh-point-steps/s "
+            f"({tuned['layouts_changed']} of {len(compiled)} layouts changed, tuning {tune_s:.1f} s)")
 
     prof = b["prof"]
     l_avg = prof["lphase_ms"] / max(1, prof["lphase_launches"])
@@ -487,6 +509,8 @@ def main():
         "line_cn": line,
         "served_stream": served,
     }
+    if tuned is not None:
+        out["autotuned"] = tuned
     print(json.dumps(out), flush=True)
     if args.calibrate_copy:  # after the JSON line: a known-size kernel for the counter passes
         calibrate_copy()

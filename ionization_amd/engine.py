@@ -16,6 +16,7 @@ reference's natural order.
 """
 
 import ctypes
+import dataclasses
 from dataclasses import dataclass, field
 from typing import Callable, Optional, Sequence, Tuple
 
@@ -36,6 +37,34 @@ LINE_NTAB = 5
 # ----------------------------------------------------------------- layout
 
 
+# Measured SH layouts by radial size class (ionization_amd.autotune): consulted
+# by choose_layout for kind "sh" when no rows are forced.
+_SH_TUNED: dict = {}
+
+
+def sh_size_class(n: int) -> int:
+    """Radial sizes share a tuned layout in classes of 128 points (the class is its upper end)."""
+    return 128 * -(-int(n) // 128)
+
+
+def set_sh_layout_table(table) -> None:
+    """Install {size class: (P, m)} measured by the autotuner (an empty table restores the rule)."""
+    _SH_TUNED.clear()
+    for k, (P, m) in dict(table).items():
+        _SH_TUNED[int(k)] = (int(P), int(m))
+
+
+def sh_layout_table() -> dict:
+    return dict(_SH_TUNED)
+
+
+def check_layout(n: int, P: int, m: int, kind: str = "sh") -> None:
+    """Raise unless (P, m) is a layout the kernels support for a line of n points."""
+    limits = {"sh": {8: 512, 16: 256}, "cylz": {8: 512, 16: 256}, "line": {8: 512, 16: 512}}[kind]
+    if m < 2 or m > 16 or m % 2 or P < 64 or P % 64 or P * m < n or P > (limits[8] if m <= 8 else limits[16]):
+        raise DeviceError(f"unsupported {kind} layout P={P}, m={m} for a line of {n} points")
+
+
 def choose_layout(n: int, kind: str = "sh", rows: Optional[int] = None) -> Tuple[int, int]:
     """
     Pick (P, m) for a line of n points: P a multiple of 64, m even, P m >= n,
@@ -47,6 +76,10 @@ def choose_layout(n: int, kind: str = "sh", rows: Optional[int] = None) -> Tuple
     a shorter reduced system and more work per thread). Line and cylindrical
     z-lines: least padding, then m = 8, then the most rows.
     """
+    if kind == "sh" and rows is None and _SH_TUNED:
+        tuned = _SH_TUNED.get(sh_size_class(n))
+        if tuned is not None and tuned[0] * tuned[1] >= n:
+            return tuned
     limits = {"sh": {8: 512, 16: 256}, "cylz": {8: 512, 16: 256}, "line": {8: 512, 16: 512}}[kind]
     cands = [rows] if rows is not None else [16, 14, 12, 10, 8, 6, 4, 2]
     best = None
@@ -203,14 +236,19 @@ def sh_radial_tables(*, r_bound, r_points, potential, mask, time_step, test_char
 def compile_sh(*, r_bound, r_points, l_bound, potential, field: Callable, mask, times, time_step,
                g0: np.ndarray, basis: Sequence[Tuple[np.ndarray, int]] = (), store_every=1,
                test_charge=u.electron_charge, hydrogen_correction=True, rows=None,
-               norm_by_l=False) -> CompiledSH:
+               norm_by_l=False, layout: Optional[Tuple[int, int]] = None) -> CompiledSH:
     """
     Compile a length-gauge SphericalHarmonicMesh + SplitInteractionOperator
     simulation. ``times`` is the reference time grid (sims.py:198-205);
     step n uses tau_n = (t_{n+1} - t_n) / 2 hbar and the field at
-    t_{n+1} + time_step / 2 (mesh_operators.py:1011-1013).
+    t_{n+1} + time_step / 2 (mesh_operators.py:1011-1013). ``layout`` = (P, m)
+    forces the device layout (the autotuner measures candidates this way).
     """
-    P, m = choose_layout(r_points, "sh", rows)
+    if layout is not None:
+        P, m = int(layout[0]), int(layout[1])
+        check_layout(r_points, P, m, "sh")
+    else:
+        P, m = choose_layout(r_points, "sh", rows)
     tables, corr0, delta_r, _, centrifugal = sh_radial_tables(
         r_bound=r_bound, r_points=r_points, potential=potential, mask=mask, time_step=time_step,
         test_charge=test_charge, hydrogen_correction=hydrogen_correction, P=P, m=m,
@@ -236,6 +274,18 @@ def compile_sh(*, r_bound, r_points, l_bound, potential, field: Callable, mask, 
         g0=np.ascontiguousarray(to_positions(g0, P, m)),
         basis=np.ascontiguousarray(to_positions(bmat, P, m)), basis_l=bl, norm_by_l=bool(norm_by_l),
     )
+
+
+def relayout_sh(c: CompiledSH, layout: Optional[Tuple[int, int]] = None) -> CompiledSH:
+    """The same simulation under another device layout (the tuned or rule one when None):
+    position-ordered arrays are permuted, nothing is recomputed."""
+    P, m = layout if layout is not None else choose_layout(c.r_points, "sh")
+    check_layout(c.r_points, P, m, "sh")
+    if (P, m) == (c.P, c.m):
+        return c
+    n = c.r_points
+    move = lambda a: np.ascontiguousarray(to_positions(from_positions(a, n, c.P, c.m), P, m))
+    return dataclasses.replace(c, P=P, m=m, tables=move(c.tables), g0=move(c.g0), basis=move(c.basis))
 
 
 @dataclass
