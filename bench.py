@@ -14,7 +14,9 @@ wavefunction resident in HBM; ``roofline`` is for the kernel with the largest
 share of device time, from HIP events recorded on the library's stream
 around every launch of a profiled pass over K more steps; ``cpu_baseline``
 is the CPU oracle (a restatement of the reference's evolve step) timed on
-one host core for a bounded sample of the same workload.
+one host core for a bounded sample of the same workload; ``cpu_baseline_parallel``
+runs it in --cpu-procs processes at once, one scan simulation each (the
+reference runs one simulation per process), before this process touches the GPU.
 """
 
 import argparse
@@ -230,6 +232,27 @@ def cpu_baseline(params, seconds):
                 sample=f"scan simulation 0 ({p['l_bound']} l x {p['r_points']} r), {steps} steps")
 
 
+def _cpu_worker(args):
+    params, i, seconds = args
+    c = cpu_baseline([params[i]], seconds)
+    return c["value"]
+
+
+def cpu_baseline_parallel(params, seconds, procs):
+    """The oracle in `procs` forked processes at once, each on its own scan simulation (the
+    reference runs one simulation per process); run before this process touches the GPU."""
+    import multiprocessing as mp
+
+    from ionization_amd import _lib
+
+    if _lib._LIB is not None:
+        raise RuntimeError("the parallel CPU baseline must run before the device library is loaded")
+    with mp.get_context("fork").Pool(procs) as pool:
+        rates = pool.map(_cpu_worker, [(params, i % len(params), seconds) for i in range(procs)])
+    return dict(value=float(sum(rates)), procs=procs,
+                sample=f"scan simulations 0..{procs - 1}, one per process, ~{seconds:.0f} s each")
+
+
 def calibrate_copy():
     """The library's streaming copy of COPY_CAL_BYTES (k_stream_copy): the
     known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters to bytes;
@@ -380,6 +403,8 @@ def main():
     ap.add_argument("--store-every", type=int, default=100)
     ap.add_argument("--plain-steps", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-procs", type=int, default=16,
+                    help="processes for the parallel CPU baseline (the host share of one GPU)")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="also measure every candidate SH layout per radial size class and time the scan "
@@ -408,6 +433,12 @@ def main():
     params, compiled = build_scan(args.sims, args.seed, args.store_every)
     log(f"built {len(compiled)} simulations in {time.perf_counter() - t:.1f} s; "
         f"{sum(c.mesh_points for c in compiled) / 1e6:.1f} M mesh points")
+    cpu_par = None
+    if args.cpu_seconds > 0 and args.cpu_procs > 1:  # before any device work: the workers are forked
+        cp = cpu_baseline_parallel(params, min(args.cpu_seconds, 10.0), args.cpu_procs)
+        cpu_par = {"value": cp["value"], "unit": "mesh-point-steps/s", "cores": cp["procs"], "kind": "port",
+                   "sample": cp["sample"]}
+        log(f"cpu oracle, {cp['procs']} processes: {cp['value'] / 1e6:.2f} M mesh-point-steps/s")
     b = time_batched(compiled, args.steps, args.warmup)
     rate = b["points"] * args.steps / b["wall_s"]
     log(f"batched: {b['wall_s'] * 1e3 / args.steps:.3f} ms/step, {rate / 1e9:.2f} G mesh-point-steps/s")
@@ -505,6 +536,7 @@ def main():
             "copy_gbs": copy_gbs,
         },
         "cpu_baseline": cpu,
+        "cpu_baseline_parallel": cpu_par,
         "cyl_adi": cyl,
         "line_cn": line,
         "served_stream": served,
