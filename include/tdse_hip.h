@@ -88,7 +88,7 @@ int tdse_tdma_batched(tdse_ctx *ctx, const double *sub, const double *diag, cons
 typedef struct {
     int32_t r_points;      /* n */
     int32_t l_bound;       /* L */
-    int32_t chunks;        /* P: threads per l-line, multiple of 64 */
+    int32_t chunks;        /* P: threads per l-line, multiple of 64, <= 512 */
     int32_t rows;          /* m: rows per thread, even, 2 <= m <= 16 */
     int32_t n_steps;       /* N = len(times) - 1 */
     int32_t store_every;   /* data cadence (sims.py:84-95); -1 = first and last only */

@@ -36,7 +36,7 @@ def sh_candidates(n: int, max_extra: float = 0.10, limit: int = 8, pow2: bool = 
     (whole waves per SIMD: in a mixed batch 3-wave workgroups measured slow)."""
     out = []
     for m in range(2, 17, 2):
-        cap = 512 if m <= 8 else 256
+        cap = 512
         P = 64 * -(-n // (64 * m))
         while P <= cap:
             if not pow2 or (P & (P - 1)) == 0:

@@ -360,10 +360,11 @@ void dispatch_rphase(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
         case 4: launch_rphase<4, 512, 4>(ctx, b, st); break;
         case 6: launch_rphase<6, 512, 4>(ctx, b, st); break;
         case 8: launch_rphase<8, 512, 4>(ctx, b, st); break;
-        case 10: launch_rphase<10, 256, 3>(ctx, b, st); break;
-        case 12: launch_rphase<12, 256, 2>(ctx, b, st); break;
-        case 14: launch_rphase<14, 256, 2>(ctx, b, st); break;
-        default: launch_rphase<16, 256, 1>(ctx, b, st); break;
+        // radial lines past 4096 points: 512-thread workgroups of deep tiles
+        case 10: b.P > 256 ? launch_rphase<10, 512, 1>(ctx, b, st) : launch_rphase<10, 256, 3>(ctx, b, st); break;
+        case 12: b.P > 256 ? launch_rphase<12, 512, 1>(ctx, b, st) : launch_rphase<12, 256, 2>(ctx, b, st); break;
+        case 14: b.P > 256 ? launch_rphase<14, 512, 1>(ctx, b, st) : launch_rphase<14, 256, 2>(ctx, b, st); break;
+        default: b.P > 256 ? launch_rphase<16, 512, 1>(ctx, b, st) : launch_rphase<16, 256, 1>(ctx, b, st); break;
     }
 }
 
@@ -738,7 +739,7 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
     if (D.r_points < 2 || D.l_bound < 1 || D.l_bound > kMaxL) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: bad mesh size");
     if (P < TDSE_WAVE || P % TDSE_WAVE) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: chunks must be a positive multiple of 64");
     if (m < 2 || m > 16 || (m & 1)) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: rows must be even and in [2, 16]");
-    if ((m <= 8 && P > 512) || (m > 8 && P > 256)) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: too many chunks for rows");
+    if (P > 512) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: at most 512 chunks");
     if ((int64_t)P * m < D.r_points) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: chunks * rows < r_points");
     if (D.n_steps < 1) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: n_steps must be >= 1");
     if (D.store_every == 0 || D.store_every < -1) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: store_every must be >= 1 or -1");

@@ -60,7 +60,7 @@ def sh_layout_table() -> dict:
 
 def check_layout(n: int, P: int, m: int, kind: str = "sh") -> None:
     """Raise unless (P, m) is a layout the kernels support for a line of n points."""
-    limits = {"sh": {8: 512, 16: 256}, "cylz": {8: 512, 16: 256}, "line": {8: 512, 16: 512}}[kind]
+    limits = {"sh": {8: 512, 16: 512}, "cylz": {8: 512, 16: 256}, "line": {8: 512, 16: 512}}[kind]
     if m < 2 or m > 16 or m % 2 or P < 64 or P % 64 or P * m < n or P > (limits[8] if m <= 8 else limits[16]):
         raise DeviceError(f"unsupported {kind} layout P={P}, m={m} for a line of {n} points")
 
@@ -68,8 +68,8 @@ def check_layout(n: int, P: int, m: int, kind: str = "sh") -> None:
 def choose_layout(n: int, kind: str = "sh", rows: Optional[int] = None) -> Tuple[int, int]:
     """
     Pick (P, m) for a line of n points: P a multiple of 64, m even, P m >= n,
-    within the kernel limits (m <= 8 -> P <= 512, m <= 16 -> P <= 256; Line:
-    P <= 512). SH radial lines: P a power of two where one fits (workgroups of
+    within the kernel limits (P <= 512, m <= 16: lines up to 8192 points; the
+    cylinder z-sweep: m <= 8 -> P <= 512, m <= 16 -> P <= 256). SH radial lines: P a power of two where one fits (workgroups of
     1, 2 or 4 waves per SIMD set; on the 256-simulation scan 3-wave workgroups
     of P = 192 made the R-phase slower: 1.65 -> 1.59 ms/step without them),
     then least padding, then the most rows per thread (fewer, longer chunks:
@@ -80,7 +80,7 @@ def choose_layout(n: int, kind: str = "sh", rows: Optional[int] = None) -> Tuple
         tuned = _SH_TUNED.get(sh_size_class(n))
         if tuned is not None and tuned[0] * tuned[1] >= n:
             return tuned
-    limits = {"sh": {8: 512, 16: 256}, "cylz": {8: 512, 16: 256}, "line": {8: 512, 16: 512}}[kind]
+    limits = {"sh": {8: 512, 16: 512}, "cylz": {8: 512, 16: 256}, "line": {8: 512, 16: 512}}[kind]
     cands = [rows] if rows is not None else [16, 14, 12, 10, 8, 6, 4, 2]
     best = None
     for m in cands:

@@ -21,7 +21,7 @@ def test_candidates_are_supported_and_least_padded_first(n):
 
 
 def test_check_layout_rejects_unsupported():
-    for P, m in [(64, 3), (96, 4), (512, 10), (64, 2), (64, 18)]:
+    for P, m in [(64, 3), (96, 4), (576, 10), (1024, 16), (64, 2), (64, 18)]:
         with pytest.raises(DeviceError):
             engine.check_layout(700, P, m, "sh")
     engine.check_layout(700, 64, 12, "sh")

@@ -65,3 +65,20 @@ def test_cyl_edge_sizes(device_ctx, z_points, rho_points):
     assert rel_l2(g_dev, g) <= WF_RTOL
     assert np.max(np.abs(norms_dev - norms)) <= OBS_ATOL
     assert np.max(np.abs(ips_dev - ips)) <= OBS_ATOL
+
+
+@pytest.mark.parametrize("r_points", [4097, 6000, 7000, 8192])
+def test_sh_radial_lines_past_4096_points(device_ctx, r_points):
+    """512-thread R-phase workgroups of 10-16 rows per thread (m = 10, 12, 14, 16)."""
+    case = sh_case(r_points=r_points, l_bound=5, t_half_as=12.0, n_basis_l=1, r_bound_bohr=r_points / 20)
+    comp = case["compiled"]
+    print(f"SH long line r_points={r_points} P={comp.P} m={comp.m}", flush=True)
+    assert comp.P == 512 and comp.m >= 10
+    s = device_ctx.add_sh(comp)
+    device_ctx.run(comp.n_steps + 1)
+    res = device_ctx.download_sh(s)
+    device_ctx.remove(engine.KIND_SH, s)
+    g, norms, ips, _ = case["oracle"].run(case["g0"], case["times"], case["dt"], 1, basis=case["basis"])
+    assert rel_l2(res.g, g) <= WF_RTOL
+    assert np.max(np.abs(res.norms - norms)) <= OBS_ATOL
+    assert np.max(np.abs(res.ips - ips)) <= OBS_ATOL
