@@ -425,8 +425,9 @@ def main():
                            args.steps, args.warmup, args.cyl_steps)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (simulations shard by "
-              "simulation with no exchange, see the design notes)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: simulations are "
+              "independent, so several GPUs would each run their own context over a share of the "
+              "simulations with no exchange, and the whole-job rate is the sum", file=sys.stderr)
         sys.exit(2)
 
     t = time.perf_counter()
