@@ -74,10 +74,13 @@ __device__ __forceinline__ int sh_data_slot(int n, int N, int every) {
     return n == 0 ? 0 : 1;
 }
 
-// exp(-i a sigma_x) applied to (x0, x1):  y0 = cos a x0 - i sin a x1,  y1 = -i sin a x0 + cos a x1
+// exp(-i a sigma_x) applied to (x0, x1):  y0 = cos a x0 - i sin a x1,  y1 = -i sin a x0 + cos a x1.
+// When every lane's angle is small (a weak field: most steps away from a
+// pulse's peak) the whole wave takes the reduction-free series.
 __device__ __forceinline__ void sh_rotate(double a, double2& x0, double2& x1) {
     double s, c;
-    tdse_sincos(a, &s, &c);
+    if (__all(fabs(a) <= kSmallAngle)) tdse_sincos_small(a, &s, &c);
+    else tdse_sincos(a, &s, &c);
     const double2 y0 = cmk(fma(c, x0.x, s * x1.y), fma(c, x0.y, -s * x1.x));
     const double2 y1 = cmk(fma(c, x1.x, s * x0.y), fma(c, x1.y, -s * x0.x));
     x0 = y0;
