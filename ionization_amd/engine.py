@@ -69,12 +69,13 @@ def choose_layout(n: int, kind: str = "sh", rows: Optional[int] = None) -> Tuple
     """
     Pick (P, m) for a line of n points: P a multiple of 64, m even, P m >= n,
     within the kernel limits (P <= 512, m <= 16: lines up to 8192 points; the
-    cylinder z-sweep: m <= 8 -> P <= 512, m <= 16 -> P <= 256). SH radial lines: P a power of two where one fits (workgroups of
-    1, 2 or 4 waves per SIMD set; on the 256-simulation scan 3-wave workgroups
-    of P = 192 made the R-phase slower: 1.65 -> 1.59 ms/step without them),
-    then least padding, then the most rows per thread (fewer, longer chunks:
-    a shorter reduced system and more work per thread). Line and cylindrical
-    z-lines: least padding, then m = 8, then the most rows.
+    cylinder z-sweep: m <= 8 -> P <= 512, m <= 16 -> P <= 256).
+    SH radial lines: P a power of two where one fits (on the 256-simulation
+    scan 3-wave workgroups of P = 192 made the R-phase slower: 1.65 -> 1.59
+    ms/step without them), then least padding, then the most rows per thread
+    (fewer, longer chunks: a shorter reduced system and more work per thread);
+    a table measured by the autotuner takes precedence when installed.
+    Line and cylindrical z-lines: least padding, then m = 8, then the most rows.
     """
     if kind == "sh" and rows is None and _SH_TUNED:
         tuned = _SH_TUNED.get(sh_size_class(n))
