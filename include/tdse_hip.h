@@ -98,7 +98,8 @@ typedef struct {
     double corr0;          /* tau0 (-2 P_r) dr (1 + dr) / 8 (mesh_operators.py:901-903) */
     double centrifugal;    /* tau0 hbar^2 / (2 mu): the U table is centrifugal / r_j^2 */
     int32_t norm_by_l;     /* 1: also reduce the norm in every l at data times (NormBySphericalHarmonic) */
-    int32_t reserved;
+    int32_t gauge;         /* 0: length gauge; 1: velocity gauge (SphericalHarmonicVelocityGaugeOperators,
+                              mesh_operators.py:1130-1408): r_points >= 3, l_bound >= 2 */
 } tdse_sh_desc;
 
 /*
@@ -106,6 +107,11 @@ typedef struct {
  * 6 * npad doubles (T, U, A_{j-1}, A_j, tau0 (-q) r, mask), fs / ts: N
  * doubles (field at t_{n+1} + dt/2 times tau_n / tau0, and tau_n / tau0),
  * g0: 2 * L * npad doubles, basis: n_basis * npad doubles, basis_l: n_basis.
+ * Velocity gauge (gauge = 1): row 4 holds tau0 hbar (q / m) / r_j (the h1
+ * factor; the library multiplies by c_l (l + 1)), a seventh row holds
+ * tau0 hbar q / (2 m dr) alpha_j for j < n - 1 and 0 beyond (the h2 factor;
+ * times c_l), and fs holds the vector potential A(t_{n+1}) times tau_n / tau0
+ * (mesh_operators.py:1143-1199).
  * Replaces MeshSpecification.to_sim() + the first call of
  * SplitInteractionOperator.evolve (sims.py:46-118, evolution_methods.py:89-123).
  */

@@ -33,7 +33,7 @@ class ShDesc(ctypes.Structure):
         ("corr0", ctypes.c_double),
         ("centrifugal", ctypes.c_double),
         ("norm_by_l", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("gauge", ctypes.c_int32),
     ]
 
 

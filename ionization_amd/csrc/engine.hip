@@ -102,6 +102,7 @@ class Arena {
 
 struct ShHost {
     bool live = false;
+    bool vel = false;    // velocity gauge (one more table row, h2 factors around the R-phase)
     ShSim dev{};
     int n_data = 0;
     int wave_first = 0;  // this simulation's range in the wave table
@@ -166,7 +167,10 @@ struct tdse_ctx {
     std::vector<Bucket> cyl_z_buckets;
     std::vector<Bucket> cyl_r_buckets;
     double* d_cl = nullptr;
-    DevVec<ShWave> waves;
+    double* d_clv = nullptr;  // c_l (l + 1): the velocity gauge's h1 coupling per l
+    DevVec<ShWave> waves;     // length-gauge simulations' waves first, then velocity-gauge ones
+    int n_len_waves = 0;
+    std::vector<Bucket> sh_vbuckets[2];  // velocity-gauge l-pairs (slot, l) of even / odd l, by (rows, chunks)
     DevVec<int2> jobs;
     std::vector<Bucket> sh_buckets;
     std::vector<int> sh_order;  // sh_buckets by decreasing work
@@ -234,11 +238,15 @@ void drop_graph(tdse_ctx* ctx) {
 int rebuild_tables(tdse_ctx* ctx) {
     std::vector<ShWave> waves;
     std::vector<int2> jobs;
+    ctx->n_len_waves = -1;
     std::map<std::pair<int, int>, std::vector<int2>> shb, lnb;  // (rows, chunks [+ 4096 if complex steps])
+    std::map<std::pair<int, int>, std::vector<int2>> vpb[2];    // velocity l-pairs by parity, (rows, chunks)
     ctx->sh_norm_by_l = false;
+    for (int pass = 0; pass < 2; ++pass)
     for (int s = 0; s < (int)ctx->sh.size(); ++s) {
         ShHost& h = ctx->sh[s];
-        if (!h.live) continue;
+        if (!h.live || (int)h.vel != pass) continue;
+        if (pass == 1 && ctx->n_len_waves < 0) ctx->n_len_waves = (int)waves.size();
         h.wave_first = (int)waves.size();
         const int nseg = sh_segments(h.dev.L);
         const int seg = (h.dev.L + nseg - 1) / nseg;
@@ -256,7 +264,10 @@ int rebuild_tables(tdse_ctx* ctx) {
         for (int b = 0; b < h.dev.n_basis; ++b) jobs.push_back(make_int2(s, b));
         auto& v = shb[{h.dev.m, h.dev.P}];
         for (int l = 0; l < h.dev.L; ++l) v.push_back(make_int2(s, l));
+        if (h.vel)
+            for (int l = 0; l + 1 < h.dev.L; ++l) vpb[l & 1][{h.dev.m, h.dev.P}].push_back(make_int2(s, l));
     }
+    if (ctx->n_len_waves < 0) ctx->n_len_waves = (int)waves.size();
     for (int s = 0; s < (int)ctx->line.size(); ++s) {
         const LineHost& h = ctx->line[s];
         if (!h.live) continue;
@@ -296,6 +307,8 @@ int rebuild_tables(tdse_ctx* ctx) {
         return TDSE_OK;
     };
     if ((rc = fill(ctx->sh_buckets, shb))) return rc;
+    if ((rc = fill(ctx->sh_vbuckets[0], vpb[0]))) return rc;
+    if ((rc = fill(ctx->sh_vbuckets[1], vpb[1]))) return rc;
     if ((rc = fill(ctx->line_buckets, lnb))) return rc;
     if ((rc = fill(ctx->cyl_z_buckets, czb))) return rc;
     if ((rc = fill(ctx->cyl_r_buckets, crb))) return rc;
@@ -376,6 +389,24 @@ void dispatch_rphase(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
         case 12: b.P > 256 ? launch_rphase<12, 512, 1>(ctx, b, st) : launch_rphase<12, 256, 2>(ctx, b, st); break;
         case 14: b.P > 256 ? launch_rphase<14, 512, 1>(ctx, b, st) : launch_rphase<14, 256, 2>(ctx, b, st); break;
         default: b.P > 256 ? launch_rphase<16, 512, 1>(ctx, b, st) : launch_rphase<16, 256, 1>(ctx, b, st); break;
+    }
+}
+
+template <bool REV>
+void dispatch_vphase(tdse_ctx* ctx, const Bucket& b) {
+    const dim3 grid((unsigned)b.lines.count), block(b.P);
+    auto args = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, ctx->d_sh, b.lines.ptr, (int)b.lines.count, ctx->d_cl);
+    };
+    switch (b.M) {
+        case 2: args(k_sh_vphase<2, REV>); break;
+        case 4: args(k_sh_vphase<4, REV>); break;
+        case 6: args(k_sh_vphase<6, REV>); break;
+        case 8: args(k_sh_vphase<8, REV>); break;
+        case 10: args(k_sh_vphase<10, REV>); break;
+        case 12: args(k_sh_vphase<12, REV>); break;
+        case 14: args(k_sh_vphase<14, REV>); break;
+        default: args(k_sh_vphase<16, REV>); break;
     }
 }
 
@@ -464,12 +495,23 @@ int launch_step(tdse_ctx* ctx, bool cyl_pre) {
     if (nw > 0) {
         prof_begin(ctx, 0);
         // the instance that reduces the norm in every l needs more registers: only when asked for
-        auto lphase = ctx->sh_norm_by_l ? k_sh_lphase<true> : k_sh_lphase<false>;
-        hipLaunchKernelGGL(lphase, dim3((nw * TDSE_WAVE + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh,
-                           ctx->waves.ptr, nw, ctx->d_cl, 0);
+        const int nl = ctx->n_len_waves, nv = nw - nl;
+        if (nl > 0) {
+            auto lphase = ctx->sh_norm_by_l ? k_sh_lphase<true, false> : k_sh_lphase<false, false>;
+            hipLaunchKernelGGL(lphase, dim3((nl * TDSE_WAVE + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh,
+                               ctx->waves.ptr, nl, ctx->d_cl, 0);
+        }
+        if (nv > 0) {
+            auto lphase = ctx->sh_norm_by_l ? k_sh_lphase<true, true> : k_sh_lphase<false, true>;
+            hipLaunchKernelGGL(lphase, dim3((nv * TDSE_WAVE + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh,
+                               ctx->waves.ptr + nl, nv, ctx->d_clv, 0);
+        }
         prof_end(ctx);
         const int nj = (int)ctx->jobs.count;
         hipLaunchKernelGGL(k_sh_observe, dim3(nj), dim3(256), 0, ctx->stream, ctx->d_sh, ctx->jobs.ptr, nj);
+        // velocity gauge, before the CN solve: h2 factors of the even l-pairs, then of the odd ones
+        for (int par = 0; par < 2; ++par)
+            for (auto& b : ctx->sh_vbuckets[par]) dispatch_vphase<false>(ctx, b);
         if (ctx->profiling || ctx->sh_buckets.size() < 2) {
             for (auto& b : ctx->sh_buckets) {
                 prof_begin(ctx, 1);
@@ -491,6 +533,9 @@ int launch_step(tdse_ctx* ctx, bool cyl_pre) {
                 HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join[j], 0));
             }
         }
+        // velocity gauge, after the CN solve: the same factors in reverse order
+        for (int par = 1; par >= 0; --par)
+            for (auto& b : ctx->sh_vbuckets[par]) dispatch_vphase<true>(ctx, b);
         const int ns = (int)ctx->sh.size();
         hipLaunchKernelGGL(k_sh_advance, dim3((ns + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh, ns);
     }
@@ -603,10 +648,15 @@ int tdse_ctx_create(int device, size_t pool_bytes, tdse_ctx** out) {
         delete ctx;
         return fail(nullptr, TDSE_ERR_NOMEM, "could not reserve the device memory pool");
     }
-    std::vector<double> cl(kMaxL);
-    for (int l = 0; l < kMaxL; ++l) cl[l] = (l + 1.0) / std::sqrt((2.0 * l + 1.0) * (2.0 * l + 3.0));
+    std::vector<double> cl(kMaxL), clv(kMaxL);
+    for (int l = 0; l < kMaxL; ++l) {
+        cl[l] = (l + 1.0) / std::sqrt((2.0 * l + 1.0) * (2.0 * l + 3.0));
+        clv[l] = cl[l] * (l + 1.0);
+    }
     bool ok = hipMalloc(&ctx->d_cl, kMaxL * sizeof(double)) == hipSuccess &&
               hipMemcpy(ctx->d_cl, cl.data(), kMaxL * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMalloc(&ctx->d_clv, kMaxL * sizeof(double)) == hipSuccess &&
+              hipMemcpy(ctx->d_clv, clv.data(), kMaxL * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
               hipMalloc(&ctx->d_sh, kSlotCapacity * sizeof(ShSim)) == hipSuccess &&
               hipMalloc(&ctx->d_line, kSlotCapacity * sizeof(LineSim)) == hipSuccess &&
               hipMalloc(&ctx->d_cyl, kSlotCapacity * sizeof(CylSim)) == hipSuccess &&
@@ -633,6 +683,9 @@ int tdse_ctx_destroy(tdse_ctx* ctx) {
     drop_graph(ctx);
     for (auto& b : ctx->sh_buckets)
         if (b.lines.ptr) (void)hipFree(b.lines.ptr);
+    for (int par = 0; par < 2; ++par)
+        for (auto& b : ctx->sh_vbuckets[par])
+            if (b.lines.ptr) (void)hipFree(b.lines.ptr);
     for (auto& b : ctx->line_buckets)
         if (b.lines.ptr) (void)hipFree(b.lines.ptr);
     for (auto& b : ctx->cyl_z_buckets)
@@ -644,6 +697,7 @@ int tdse_ctx_destroy(tdse_ctx* ctx) {
     if (ctx->waves.ptr) (void)hipFree(ctx->waves.ptr);
     if (ctx->jobs.ptr) (void)hipFree(ctx->jobs.ptr);
     if (ctx->d_cl) (void)hipFree(ctx->d_cl);
+    if (ctx->d_clv) (void)hipFree(ctx->d_clv);
     if (ctx->d_sh) (void)hipFree(ctx->d_sh);
     if (ctx->d_line) (void)hipFree(ctx->d_line);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -765,6 +819,9 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
     if (D.store_every == 0 || D.store_every < -1) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: store_every must be >= 1 or -1");
     if (D.n_basis < 0 || (D.n_basis > 0 && (!basis || !basis_l))) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: basis missing");
     if (D.lb < 1 || D.lb > D.l_bound) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: lb out of range");
+    if (D.gauge != 0 && D.gauge != 1) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: gauge must be 0 (length) or 1 (velocity)");
+    if (D.gauge == 1 && (D.r_points < 3 || D.l_bound < 2))
+        return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: the velocity gauge needs r_points >= 3 and l_bound >= 2");
     for (int b = 0; b < D.n_basis; ++b)
         if (basis_l[b] < 0 || basis_l[b] >= D.lb) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: basis l must be < lb");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -781,6 +838,7 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
     }
     ShHost& H = ctx->sh[s];
     H.blocks.clear();
+    H.vel = D.gauge == 1;
     ShSim& S = H.dev;
     S = ShSim{};
     S.n = D.r_points;
@@ -810,7 +868,7 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
     if ((rc = copy_to_device(ctx, H.blocks, reinterpret_cast<const double2*>(g0), (size_t)S.L * npad, &g)) ||
         (rc = copy_to_device(ctx, H.blocks, ta_host.data(), npad, &ta)) ||
         (rc = copy_to_device<double2>(ctx, H.blocks, nullptr, (size_t)S.L * npad, &g2)) ||
-        (rc = copy_to_device(ctx, H.blocks, tables, SH_NTAB * npad, &tab)) ||
+        (rc = copy_to_device(ctx, H.blocks, tables, (SH_NTAB + (H.vel ? 1 : 0)) * npad, &tab)) ||
         (rc = copy_to_device(ctx, H.blocks, fs, (size_t)S.N, &dfs)) ||
         (rc = copy_to_device(ctx, H.blocks, ts, (size_t)S.N, &dts)) ||
         (rc = copy_to_device<double2>(ctx, H.blocks, nullptr, (size_t)S.LB * npad, &snap)) ||
@@ -1298,8 +1356,12 @@ int tdse_materialize(tdse_ctx* ctx, int32_t kind, int32_t slot) {
     const ShHost& H = ctx->sh[slot];
     if (st[1] == 1 || st[0] == 0 || st[0] > H.dev.N) return TDSE_OK;
     const int nw = H.wave_count;
-    hipLaunchKernelGGL(k_sh_lphase<false>, dim3((nw * TDSE_WAVE + 255) / 256), dim3(256), 0, ctx->stream,
-                       ctx->d_sh, ctx->waves.ptr + H.wave_first, nw, ctx->d_cl, 1);
+    if (H.vel)
+        hipLaunchKernelGGL((k_sh_lphase<false, true>), dim3((nw * TDSE_WAVE + 255) / 256), dim3(256), 0, ctx->stream,
+                           ctx->d_sh, ctx->waves.ptr + H.wave_first, nw, ctx->d_clv, 1);
+    else
+        hipLaunchKernelGGL((k_sh_lphase<false, false>), dim3((nw * TDSE_WAVE + 255) / 256), dim3(256), 0,
+                           ctx->stream, ctx->d_sh, ctx->waves.ptr + H.wave_first, nw, ctx->d_cl, 1);
     HIP_TRY(ctx, hipGetLastError());
     // the end-only L-phase wrote sh_state(n + 1): swap the buffers so the exact
     // state is sh_state(n) for the unchanged step counter

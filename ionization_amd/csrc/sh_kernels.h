@@ -56,6 +56,8 @@ struct ShSim {
 constexpr int kShSegRows = 64;  // most rows of one L-phase wave (its l-segment)
 
 enum { SH_T = 0, SH_U = 1, SH_AP = 2, SH_AN = 3, SH_RQ = 4, SH_MASK = 5, SH_NTAB = 6 };
+// velocity-gauge simulations carry one more row: tau0 hbar q / (2 m dr) alpha_j, zero for j >= n - 1
+enum { SH_AV = 6 };
 
 // The wavefunction ping-pongs between two buffers: the state with step
 // counter n lives in sh_state(S, n). The L-phase of a step reads
@@ -74,17 +76,30 @@ __device__ __forceinline__ int sh_data_slot(int n, int N, int every) {
     return n == 0 ? 0 : 1;
 }
 
-// exp(-i a sigma_x) applied to (x0, x1):  y0 = cos a x0 - i sin a x1,  y1 = -i sin a x0 + cos a x1.
+// Length gauge (VEL = false): exp(-i a sigma_x) applied to (x0, x1),
+//     y0 = cos a x0 - i sin a x1,  y1 = -i sin a x0 + cos a x1.
+// Velocity gauge (VEL = true): the real rotation of split_h1 / split_h2
+// (mesh_operators.py:1201-1250, 1252-1408), diagonal cos a, super-diagonal
+// sin a, sub-diagonal -sin a:
+//     y0 = cos a x0 + sin a x1,    y1 = -sin a x0 + cos a x1.
 // When every lane's angle is small (a weak field: most steps away from a
 // pulse's peak) the whole wave takes the reduction-free series.
+template <bool VEL = false>
 __device__ __forceinline__ void sh_rotate(double a, double2& x0, double2& x1) {
     double s, c;
     if (__all(fabs(a) <= kSmallAngle)) tdse_sincos_small(a, &s, &c);
     else tdse_sincos(a, &s, &c);
-    const double2 y0 = cmk(fma(c, x0.x, s * x1.y), fma(c, x0.y, -s * x1.x));
-    const double2 y1 = cmk(fma(c, x1.x, s * x0.y), fma(c, x1.y, -s * x0.x));
-    x0 = y0;
-    x1 = y1;
+    if constexpr (VEL) {
+        const double2 y0 = cmk(fma(c, x0.x, s * x1.x), fma(c, x0.y, s * x1.y));
+        const double2 y1 = cmk(fma(c, x1.x, -s * x0.x), fma(c, x1.y, -s * x0.y));
+        x0 = y0;
+        x1 = y1;
+    } else {
+        const double2 y0 = cmk(fma(c, x0.x, s * x1.y), fma(c, x0.y, -s * x1.x));
+        const double2 y1 = cmk(fma(c, x1.x, s * x0.y), fma(c, x1.y, -s * x0.x));
+        x0 = y0;
+        x1 = y1;
+    }
 }
 
 // ------------------------------------------------------------------ L-phase
@@ -115,7 +130,12 @@ struct ShWave {
 
 // BY_L: the instance that can also reduce the norm in every l (launched only
 // while some simulation asks for it: the reductions cost registers).
-template <bool BY_L>
+// VEL: velocity-gauge simulations. Their h1 coupling has the same pairs and
+// the same place in the step as the length gauge's E / O (split_h1,
+// mesh_operators.py:1201-1250): the host supplies the angle factors (RQ row:
+// tau0 hbar (q / m) / r_j; fs: the vector potential; cl: c_l (l + 1)) and the
+// rotation is the real one.
+template <bool BY_L, bool VEL>
 __global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, const ShWave* __restrict__ waves,
                                                    int n_waves, const double* __restrict__ cl, int end_only) {
     constexpr int PF = 4;  // prefetch depth in iterations (2 rows each)
@@ -204,16 +224,16 @@ __global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, con
 
         // stage 1: O_end on (v0-1, v0)
         double2 s1m = carry_u, s10 = u0;
-        if (do_end) sh_rotate(f_end * c_m1, s1m, s10);
+        if (do_end) sh_rotate<VEL>(f_end * c_m1, s1m, s10);
         // stage 2: E_end on (v0-2, v0-1), mask; stage 3: E_start on the same pair
         double2 s2a = carry_s1, s2b = s1m;
         if (fuse_e) {
-            sh_rotate(f_sum * c_m2, s2a, s2b);
+            sh_rotate<VEL>(f_sum * c_m2, s2a, s2b);
             s2a = cscale(mk, s2a);
             s2b = cscale(mk, s2b);
         } else {
             if (do_end) {
-                sh_rotate(f_end * c_m2, s2a, s2b);
+                sh_rotate<VEL>(f_end * c_m2, s2a, s2b);
                 s2a = cscale(mk, s2a);
                 s2b = cscale(mk, s2b);
             }
@@ -245,11 +265,11 @@ __global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, con
                 }
                 pend = r1;
             }
-            if (do_start) sh_rotate(f_start * c_m2, s2a, s2b);
+            if (do_start) sh_rotate<VEL>(f_start * c_m2, s2a, s2b);
         }
         // stage 4: O_start on (v0-3, v0-2)
         double2 o_a = carry_s3, o_b = s2a;
-        if (do_start) sh_rotate(f_start * c_m3, o_a, o_b);
+        if (do_start) sh_rotate<VEL>(f_start * c_m3, o_a, o_b);
         const int lo_row = v0 - 3 - q;
         if (live) {
             if (lo_row >= lo && lo_row < hi) st2s(g, (size_t)lo_row * npad, o_a);
@@ -447,6 +467,109 @@ k_sh_rphase(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_line
         carry2 = tv.y;
     };
     cn_line_finish<M, FORM>(x, gl, k, P, coef, coef2, lds);
+}
+
+// ------------------------------------------------------------------ velocity gauge: h2
+//
+// The radial part h2 of the velocity-gauge interaction couples (l, j) with
+// (l +- 1, j +- 1): A(t) i hbar q / (2 m dr) kron(C, Alpha) with C the
+// symmetric l-coupling (c_l off the diagonal) and Alpha the antisymmetric
+// r-coupling (alpha_j above, -alpha_j below the diagonal)
+// (mesh_operators.py:1143-1180). split_h2 (mesh_operators.py:1252-1408)
+// factors exp(-i tau h2) into four SimilarityOperators (mesh_operators.py:
+// 150-194): for the l-pairs (l, l + 1) of one parity (even l first, then odd
+// l) it maps the pair to s = (g_l + g_{l+1}) / sqrt 2, d = (g_l - g_{l+1}) /
+// sqrt 2, which diagonalises C within the pair, rotates s by +theta_j and d
+// by -theta_j on the radial pairs (j, j + 1) of even j, then on those of odd
+// j, and maps back. theta_j = tau_n A_n hbar q / (2 m dr) c_l alpha_j (zero
+// for j >= n - 1: the pair (n - 1, n) does not exist). The reversed half of
+// the step (after the CN solve) applies the odd-j pairs first.
+//
+// One workgroup per l-pair, the R-phase layout: thread k holds radial rows
+// j = k M .. k M + M - 1 (position i P + k) of both l-rows in registers and
+// updates them in place. With M even the even-j pairs lie inside a thread;
+// of the odd-j pairs only the one across a chunk edge (k M + M - 1, (k + 1) M)
+// needs the neighbouring thread's row, exchanged through LDS (both threads
+// compute that pair's rotation from the same values and keep their half).
+// Rotate (s0, s1) by +th and (d0, d1) by -th (one sine and cosine for both).
+__device__ __forceinline__ void sh_vel_pair(double th, double2& s0, double2& s1, double2& d0, double2& d1) {
+    double sn, cs;
+    if (__all(fabs(th) <= kSmallAngle)) tdse_sincos_small(th, &sn, &cs);
+    else tdse_sincos(th, &sn, &cs);
+    const double2 a0 = cmk(fma(cs, s0.x, sn * s1.x), fma(cs, s0.y, sn * s1.y));
+    const double2 a1 = cmk(fma(cs, s1.x, -sn * s0.x), fma(cs, s1.y, -sn * s0.y));
+    const double2 b0 = cmk(fma(cs, d0.x, -sn * d1.x), fma(cs, d0.y, -sn * d1.y));
+    const double2 b1 = cmk(fma(cs, d1.x, sn * d0.x), fma(cs, d1.y, sn * d0.y));
+    s0 = a0;
+    s1 = a1;
+    d0 = b0;
+    d1 = b1;
+}
+
+template <int M>
+__device__ __forceinline__ void sh_vel_rpairs(double2 (&s)[M], double2 (&d)[M], const double* __restrict__ av,
+                                              const double f, const int P, const int k, const bool odd,
+                                              double2 (*ex)[512]) {
+    if (!odd) {
+#pragma unroll
+        for (int i = 0; i < M; i += 2) sh_vel_pair(f * ld1(av, (size_t)i * P + k), s[i], s[i + 1], d[i], d[i + 1]);
+        return;
+    }
+#pragma unroll
+    for (int i = 1; i + 1 < M; i += 2) sh_vel_pair(f * ld1(av, (size_t)i * P + k), s[i], s[i + 1], d[i], d[i + 1]);
+    ex[0][k] = s[M - 1];
+    ex[1][k] = d[M - 1];
+    ex[2][k] = s[0];
+    ex[3][k] = d[0];
+    __syncthreads();
+    // the edge pair to the right: (my row M - 1, the next thread's row 0)
+    double2 sr = (k + 1 < P) ? ex[2][k + 1] : cmk(0.0, 0.0), dr = (k + 1 < P) ? ex[3][k + 1] : cmk(0.0, 0.0);
+    const double thr = (k + 1 < P) ? f * ld1(av, (size_t)(M - 1) * P + k) : 0.0;
+    sh_vel_pair(thr, s[M - 1], sr, d[M - 1], dr);
+    // the edge pair to the left: (the previous thread's row M - 1, my row 0)
+    double2 sl = (k > 0) ? ex[0][k - 1] : cmk(0.0, 0.0), dl = (k > 0) ? ex[1][k - 1] : cmk(0.0, 0.0);
+    const double thl = (k > 0) ? f * ld1(av, (size_t)(M - 1) * P + k - 1) : 0.0;
+    sh_vel_pair(thl, sl, s[0], dl, d[0]);
+}
+
+// REV = false: the half before the CN solve (even-j pairs, then odd-j);
+// REV = true: the half after it (odd-j, then even-j). The launch covers the
+// l-pairs of one parity; the host orders the launches (even l-pairs, odd
+// l-pairs) before the solve and (odd, even) after it.
+template <int M, bool REV>
+__global__ void __launch_bounds__(512) k_sh_vphase(ShSim* __restrict__ sims, const int2* __restrict__ pairs,
+                                                   int n_pairs, const double* __restrict__ cl) {
+    __shared__ double2 ex[4][512];
+    const int pair = xcd_remap(blockIdx.x, gridDim.x);
+    if (pair >= n_pairs) return;
+    const int2 pt = pairs[pair];
+    ShSim& S = sims[pt.x];
+    const int n = S.step;
+    if (!S.active || n >= S.N) return;  // uniform across the block
+    const double fn = ld1(S.fs, n);
+    if (fn == 0.0) return;  // every angle is zero: the factor is the identity
+    const int l = pt.y;
+    const int P = S.P, npad = S.npad;
+    const int k = threadIdx.x;
+    const double f = fn * ld1(cl, l);
+    double2* __restrict__ g0 = sh_state(S, n + 1) + (size_t)l * npad + k;
+    double2* __restrict__ g1 = g0 + npad;
+    const double* __restrict__ av = S.tab + (size_t)SH_AV * npad;
+    constexpr double kRsqrt2 = 0.70710678118654752440;
+    double2 s[M], d[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        const double2 a = ld2s(g0, (size_t)i * P), b = ld2s(g1, (size_t)i * P);
+        s[i] = cmk((a.x + b.x) * kRsqrt2, (a.y + b.y) * kRsqrt2);
+        d[i] = cmk((a.x - b.x) * kRsqrt2, (a.y - b.y) * kRsqrt2);
+    }
+    sh_vel_rpairs<M>(s, d, av, f, P, k, REV, ex);
+    sh_vel_rpairs<M>(s, d, av, f, P, k, !REV, ex);
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        st2s(g0, (size_t)i * P, cmk((s[i].x + d[i].x) * kRsqrt2, (s[i].y + d[i].y) * kRsqrt2));
+        st2s(g1, (size_t)i * P, cmk((s[i].x - d[i].x) * kRsqrt2, (s[i].y - d[i].y) * kRsqrt2));
+    }
 }
 
 // Advance every active simulation's step counter after a full graph step.

@@ -23,6 +23,7 @@ from typing import Callable, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _lib
+from . import potentials
 from . import units as u
 from .exceptions import DeviceError
 
@@ -31,6 +32,7 @@ KIND_LINE = 1
 KIND_CYL = 2
 
 SH_NTAB = 6
+SH_RQ = 4  # row of the interaction's radial factor
 LINE_NTAB = 5
 
 
@@ -162,13 +164,14 @@ class CompiledSH:
     delta_r: float
     corr0: float
     centrifugal: float  # tau0 hbar^2 / (2 mu); U_j = centrifugal / r_j^2
-    tables: np.ndarray  # (6, npad)
+    tables: np.ndarray  # (6, npad); (7, npad) in the velocity gauge
     fs: np.ndarray  # (N,)
     ts: np.ndarray  # (N,)
     g0: np.ndarray  # (L, npad) complex128, position order
     basis: np.ndarray  # (nb, npad) float64, position order
     basis_l: np.ndarray  # (nb,) int32
     norm_by_l: bool = False  # also reduce the norm in every l at data times
+    gauge: str = "length"  # "length" or "velocity"
 
     @property
     def npad(self):
@@ -184,7 +187,7 @@ class CompiledSH:
         nd = data_count(self.n_steps, self.store_every)
         return int(
             2 * 16 * self.l_bound * self.npad
-            + 8 * SH_NTAB * self.npad
+            + 8 * len(self.tables) * self.npad
             + 16 * self.n_steps
             + 16 * self.lb * self.npad
             + 8 * len(self.basis_l) * self.npad
@@ -234,17 +237,59 @@ def sh_radial_tables(*, r_bound, r_points, potential, mask, time_step, test_char
     return tables, corr0, delta_r, r, centrifugal
 
 
+def vector_potential_table(field: Callable, times) -> np.ndarray:
+    """
+    A_n for every step n: the velocity gauge's step n takes
+    get_vector_potential_amplitude_numeric(sim.times_to_current) with
+    time_index already advanced, i.e. -integral of E over times[0 .. n + 1]
+    (mesh_operators.py:1182-1186, sims.py:195-196, 317-321), by the rule of
+    ``potentials.simps_cumulative``.
+    """
+    t = np.asarray(times, dtype=np.float64)
+    return -potentials.simps_cumulative(np.asarray(field(t), dtype=np.float64) * np.ones_like(t), t)[1:]
+
+
+def sh_velocity_rows(*, r_bound, r_points, time_step, test_charge, test_mass, P, m):
+    """
+    The velocity gauge's two radial factor rows in tau0 units, position order
+    (mesh_operators.py:1143-1180):
+      h1 (l coupling at fixed r):     tau0 hbar (q / m) / r_j          (times c_l (l + 1) on the device)
+      h2 (l and r coupling together): tau0 hbar q / (2 m dr) alpha_j  (times c_l), zero for j >= n - 1
+    With these, the rotation angles of split_h1 / split_h2 are -i tau (i ...) A = tau A (...)
+    (mesh_operators.py:1201-1215, 1252-1262).
+    """
+    r = np.linspace(0, r_bound, r_points)
+    delta_r = r[1] - r[0]
+    r = r + delta_r / 2
+    tau0 = time_step / (2 * u.hbar)
+    h1 = tau0 * u.hbar * (test_charge / test_mass) / r
+    j = np.arange(r_points, dtype=np.float64)
+    xa = (j ** 2) + (2 * j)
+    alpha = (xa + 1) / (xa + 0.75)
+    h2 = tau0 * u.hbar * test_charge / (test_mass * 2 * delta_r) * alpha
+    h2[-1] = 0.0
+    return to_positions(np.stack([h1, h2]), P, m)
+
+
 def compile_sh(*, r_bound, r_points, l_bound, potential, field: Callable, mask, times, time_step,
                g0: np.ndarray, basis: Sequence[Tuple[np.ndarray, int]] = (), store_every=1,
                test_charge=u.electron_charge, hydrogen_correction=True, rows=None,
-               norm_by_l=False, layout: Optional[Tuple[int, int]] = None) -> CompiledSH:
+               norm_by_l=False, layout: Optional[Tuple[int, int]] = None, gauge: str = "length",
+               test_mass=u.electron_mass_reduced) -> CompiledSH:
     """
-    Compile a length-gauge SphericalHarmonicMesh + SplitInteractionOperator
-    simulation. ``times`` is the reference time grid (sims.py:198-205);
-    step n uses tau_n = (t_{n+1} - t_n) / 2 hbar and the field at
-    t_{n+1} + time_step / 2 (mesh_operators.py:1011-1013). ``layout`` = (P, m)
-    forces the device layout (the autotuner measures candidates this way).
+    Compile a SphericalHarmonicMesh + SplitInteractionOperator simulation.
+    ``times`` is the reference time grid (sims.py:198-205); step n uses
+    tau_n = (t_{n+1} - t_n) / 2 hbar. Length gauge: the field at
+    t_{n+1} + time_step / 2 (mesh_operators.py:1011-1013). Velocity gauge
+    (``gauge="velocity"``, SphericalHarmonicVelocityGaugeOperators): the
+    vector potential of ``vector_potential_table`` and ``test_mass`` in the
+    interaction (mesh_operators.py:1143-1186). ``layout`` = (P, m) forces the
+    device layout (the autotuner measures candidates this way).
     """
+    if gauge not in ("length", "velocity"):
+        raise DeviceError(f"unknown gauge {gauge!r}")
+    if gauge == "velocity" and (r_points < 3 or l_bound < 2):
+        raise DeviceError("the velocity gauge needs r_points >= 3 and l_bound >= 2")
     if layout is not None:
         P, m = int(layout[0]), int(layout[1])
         check_layout(r_points, P, m, "sh")
@@ -257,7 +302,14 @@ def compile_sh(*, r_bound, r_points, l_bound, potential, field: Callable, mask, 
     times = np.asarray(times, dtype=np.float64)
     steps = np.diff(times)
     ts = steps / time_step
-    fs = np.asarray(field(times[1:] + time_step / 2), dtype=np.float64) * np.ones_like(ts) * ts
+    if gauge == "velocity":
+        vrows = sh_velocity_rows(r_bound=r_bound, r_points=r_points, time_step=time_step, test_charge=test_charge,
+                                 test_mass=test_mass, P=P, m=m)
+        tables = np.concatenate([tables, vrows[1:]])
+        tables[SH_RQ] = vrows[0]
+        fs = vector_potential_table(field, times) * ts
+    else:
+        fs = np.asarray(field(times[1:] + time_step / 2), dtype=np.float64) * np.ones_like(ts) * ts
     g0 = np.asarray(g0, dtype=np.complex128).reshape(l_bound, r_points)
     lb = 1 + max([l for _, l in basis], default=0)
     bmat = np.zeros((len(basis), r_points), dtype=np.float64)
@@ -274,6 +326,7 @@ def compile_sh(*, r_bound, r_points, l_bound, potential, field: Callable, mask, 
         fs=np.ascontiguousarray(fs), ts=np.ascontiguousarray(ts),
         g0=np.ascontiguousarray(to_positions(g0, P, m)),
         basis=np.ascontiguousarray(to_positions(bmat, P, m)), basis_l=bl, norm_by_l=bool(norm_by_l),
+        gauge=gauge,
     )
 
 
@@ -596,7 +649,8 @@ class DeviceContext:
     # -- simulations
     def add_sh(self, c: CompiledSH) -> int:
         desc = _lib.ShDesc(c.r_points, c.l_bound, c.P, c.m, c.n_steps, c.store_every, len(c.basis_l), c.lb,
-                           c.delta_r, c.corr0, c.centrifugal, 1 if c.norm_by_l else 0, 0)
+                           c.delta_r, c.corr0, c.centrifugal, 1 if c.norm_by_l else 0,
+                           1 if c.gauge == "velocity" else 0)
         slot = ctypes.c_int32()
         basis = np.ascontiguousarray(c.basis, dtype=np.float64) if len(c.basis_l) else None
         bl = np.ascontiguousarray(c.basis_l, dtype=np.int32)

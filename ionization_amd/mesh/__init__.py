@@ -7,6 +7,7 @@ from .mesh_operators import (
     LineLengthGaugeOperators,
     CylindricalSliceLengthGaugeOperators,
     SphericalHarmonicLengthGaugeOperators,
+    SphericalHarmonicVelocityGaugeOperators,
 )
 from .evolution_methods import EvolutionMethod, SplitInteractionOperator, AlternatingDirectionImplicit
 from .sims import (

@@ -13,9 +13,8 @@ of time.
 import copy
 
 import numpy as np
-from scipy import integrate
 
-from .. import exceptions
+from .. import exceptions, potentials
 
 
 class Data:
@@ -79,11 +78,10 @@ class Fields(Datastore):
         times = sim.data_times
         self.electric_field_amplitude = np.asarray(
             sim.spec.electric_potential.get_electric_field_amplitude(times), dtype=np.float64) * np.ones(len(times))
-        # vector potential: -integral of E from t0 to each data time on the simulation time grid
+        # vector potential: -integral of E over times_to_current at each data time (data.py:161-170)
         all_t = sim.times
         e_all = np.asarray(sim.spec.electric_potential.get_electric_field_amplitude(all_t), dtype=np.float64)
-        e_all = e_all * np.ones(len(all_t))
-        cum = np.concatenate([[0.0], integrate.cumulative_simpson(e_all, x=all_t)]) if len(all_t) > 1 else np.zeros(1)
+        cum = potentials.simps_cumulative(e_all * np.ones(len(all_t)), all_t)
         self.vector_potential_amplitude = -cum[sim.data_indices]
         super().init(sim)
 

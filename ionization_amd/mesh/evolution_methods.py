@@ -53,17 +53,20 @@ def _potential(sim):
 
 class SplitInteractionOperator(EvolutionMethod):
     """
-    Split-operator step for a SphericalHarmonicMesh in the length gauge
-    (evolution_methods.py:80-123): E, O, CN explicit, CN implicit, O, E, then
-    the mask, with the 2x2 rotations on the l-flattened pairs.
+    Split-operator step for a SphericalHarmonicMesh (evolution_methods.py:80-123):
+    the split interaction factors, CN explicit, CN implicit, the factors in
+    reverse, then the mask. Length gauge: E, O, the 2x2 rotations on the
+    l-flattened pairs. Velocity gauge: the h1 rotations on the same pairs, then
+    the four h2 similarity factors (mesh_operators.py:1188-1408).
     """
 
     def compile(self, sim, g0=None):
         mesh = sim.mesh
         if not isinstance(mesh, meshes.SphericalHarmonicMesh):
             raise NotImplementedError("SplitInteractionOperator runs on SphericalHarmonicMesh only")
-        if sim.spec.operators.gauge != core.Gauge.LENGTH:
-            raise NotImplementedError("the device split operator implements the length gauge")
+        gauge = sim.spec.operators.gauge
+        if gauge not in (core.Gauge.LENGTH, core.Gauge.VELOCITY):
+            raise NotImplementedError(f"unknown gauge {gauge}")
         basis = []
         for state in sim.spec.test_states:
             comps = list(state)
@@ -80,6 +83,7 @@ class SplitInteractionOperator(EvolutionMethod):
             hydrogen_correction=sim.spec.operators.hydrogen_zero_angular_momentum_correction,
             rows=getattr(sim.spec, "device_rows", None),
             norm_by_l=data.NormBySphericalHarmonic in getattr(sim, "datastores_by_type", {}),
+            gauge="velocity" if gauge == core.Gauge.VELOCITY else "length", test_mass=sim.spec.test_mass,
         )
 
 

@@ -95,6 +95,27 @@ class SphericalHarmonicLengthGaugeOperators(MeshOperators):
         return diag + v * np.ones_like(r), off
 
 
+class SphericalHarmonicVelocityGaugeOperators(SphericalHarmonicLengthGaugeOperators):
+    """
+    Velocity-gauge operators of a SphericalHarmonicMesh
+    (mesh_operators.py:1130-1408), for SplitInteractionOperator. H0 is the
+    length gauge's; the interaction A(t) (h1 + h2) couples neighbouring l at
+    fixed r (h1: i hbar (q / m) c_l (l + 1) / r) and neighbouring (l, r)
+    together (h2: i hbar q / (2 m dr) kron(C, Alpha)), with the vector
+    potential integrated from the field over the time grid so far
+    (``engine.vector_potential_table``) and the simulation's test_mass.
+    """
+
+    gauge = core.Gauge.VELOCITY
+
+    def __init__(self, hydrogen_zero_angular_momentum_correction: bool = True):
+        super().__init__(hydrogen_zero_angular_momentum_correction=hydrogen_zero_angular_momentum_correction)
+
+    def __repr__(self):
+        return (f"{self.__class__.__name__}("
+                f"hydrogen_zero_angular_momentum_correction={self.hydrogen_zero_angular_momentum_correction})")
+
+
 class LineLengthGaugeOperators(MeshOperators):
     """Length-gauge operators of a LineMesh (mesh_operators.py:304-349)."""
 

@@ -144,6 +144,40 @@ class SmoothedTrapezoidalWindow(TimeWindow):
 # ---------------------------------------------------------------- fields
 
 
+def simps_cumulative(y, t) -> np.ndarray:
+    """
+    The integral of samples y over t[0 .. i] for every i, each one by SciPy's
+    ``integrate.simps`` as it stood before SciPy 1.11 (default even='avg'),
+    the rule the reference's numeric vector potential is quoted with
+    (pulses.py:59-78). An odd number of samples takes composite Simpson on the
+    nonuniform grid (panels over samples (s, s + 1, s + 2) with
+    h0 = t[s + 1] - t[s], h1 = t[s + 2] - t[s + 1]); an even number averages
+    (Simpson over all but the last interval + trapezoid on it) and
+    (trapezoid on the first interval + Simpson over the rest); a single
+    sample integrates to 0. Two running sums of panels (those starting at
+    even and at odd sample indices) give every prefix in O(n).
+    """
+    t = np.asarray(t, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64) * np.ones_like(t)
+    out = np.zeros(len(t))
+    if len(t) < 2:
+        return out
+    h = np.diff(t)
+    h0, h1 = h[:-1], h[1:]
+    hsum, hprod, h0divh1 = h0 + h1, h0 * h1, h0 / h1
+    panel = hsum / 6.0 * (y[:-2] * (2 - 1.0 / h0divh1) + y[1:-1] * (hsum * hsum / hprod) + y[2:] * (2 - h0divh1))
+    cum_e = np.concatenate([[0.0], np.cumsum(panel[0::2])])  # panels starting at samples 0, 2, 4, ...
+    cum_o = np.concatenate([[0.0], np.cumsum(panel[1::2])])  # panels starting at samples 1, 3, 5, ...
+    k = np.arange(1, len(t))  # last sample index of the prefix
+    even_k = k % 2 == 0
+    out[1:][even_k] = cum_e[k[even_k] // 2]
+    ko = k[~even_k]
+    q = (ko - 1) // 2
+    val = 0.5 * (0.5 * h[ko - 1] * (y[ko] + y[ko - 1]) + 0.5 * h[0] * (y[1] + y[0]))
+    out[1:][~even_k] = 0.5 * (cum_e[q] + cum_o[q]) + val
+    return out
+
+
 class ElectricPotential(PotentialEnergy):
     pass
 
@@ -162,6 +196,14 @@ class UniformLinearlyPolarizedElectricPotential(ElectricPotential):
 
     def __call__(self, *, t, z, test_charge, **kwargs):
         return -z * test_charge * self.get_electric_field_amplitude(t)
+
+    def get_vector_potential_amplitude_numeric(self, times):
+        """-integral of E over ``times`` (pulses.py:73-78), by the rule of ``simps_cumulative``."""
+        return -simps_cumulative(self.get_electric_field_amplitude(times), times)[-1]
+
+    def get_vector_potential_amplitude_numeric_cumulative(self, times):
+        """The vector potential at every prefix of ``times``, by the same rule."""
+        return -simps_cumulative(self.get_electric_field_amplitude(times), times)
 
 
 class NoElectricPotential(UniformLinearlyPolarizedElectricPotential):

@@ -75,15 +75,75 @@ class _Op:
         self.kind, self.matrix, self.direction = kind, matrix, direction
 
 
+def similarity_transform(g2d, parity):
+    """
+    SimilarityOperator.u_even_g / u_odd_g (mesh_operators.py:166-186) on the
+    rows (l) of a wrapped (L, R) mesh: pairs of rows (a, b) become
+    (a + b, a - b) / sqrt 2; a row left without a partner (the first for odd
+    parity, the last when the count is uneven) is multiplied by sqrt 2 and
+    divided again. Returns the rows concatenated (the R-order flattening).
+    """
+    rows = list(g2d)
+    out = []
+    if parity == "odd":
+        out.append(np.sqrt(2) * rows[0])
+        rows = rows[1:]
+    for i in range(0, len(rows) - 1, 2):
+        a, b = rows[i], rows[i + 1]
+        out += [a + b, a - b]
+    if len(rows) % 2:
+        out.append(np.sqrt(2) * rows[-1])
+    return np.hstack(out) / np.sqrt(2)
+
+
 def apply_sequentially(flatten, wrap, g, ops):
-    """apply_operators_sequentially + MeshOperator.apply (mesh_operators.py:75-89, 207-218)."""
+    """apply_operators_sequentially + MeshOperator.apply (mesh_operators.py:75-89, 207-218);
+    kind 'sim' is SimilarityOperator.apply (mesh_operators.py:188-204)."""
     current = None
     for op in ops:
+        if op.kind == "sim":
+            matrix, parity = op.matrix
+            g = similarity_transform(wrap(g, current), parity)
+            g = matrix.dot(flatten(wrap(g, op.direction), op.direction))
+            g = similarity_transform(wrap(g, op.direction), parity)
+            current = op.direction
+            continue
         if current != op.direction:
             g = flatten(wrap(g, current), op.direction)
         g = op.matrix.dot(g) if op.kind == "dot" else tdma(op.matrix, g)
         current = op.direction
     return wrap(g, current)
+
+
+def simps_old(y, x):
+    """
+    scipy.integrate.simps(y, x) as it stood before SciPy 1.11 (even='avg'),
+    restated from its published algorithm: composite Simpson with the
+    nonuniform-spacing panel for an odd sample count; for an even count the
+    mean of (Simpson on samples [0, N-2] + trapezoid on the last interval) and
+    (trapezoid on the first interval + Simpson on samples [1, N-1]).
+    """
+    y = np.asarray(y, dtype=np.float64)
+    x = np.asarray(x, dtype=np.float64)
+
+    def basic(start, stop):
+        # panels starting at start, start + 2, ... < stop
+        h = np.diff(x)
+        h0 = h[start:stop:2]
+        h1 = h[start + 1:stop + 1:2]
+        y0, y1, y2 = y[start:stop:2], y[start + 1:stop + 1:2], y[start + 2:stop + 2:2]
+        m = min(len(h0), len(h1), len(y0), len(y1), len(y2))
+        h0, h1, y0, y1, y2 = h0[:m], h1[:m], y0[:m], y1[:m], y2[:m]
+        hsum, hprod = h0 + h1, h0 * h1
+        h0divh1 = h0 / h1
+        return np.sum(hsum / 6.0 * (y0 * (2 - 1.0 / h0divh1) + y1 * (hsum * hsum / hprod) + y2 * (2 - h0divh1)))
+
+    n = len(y)
+    if n % 2 == 1:
+        return basic(0, n - 2)
+    val = 0.5 * (x[-1] - x[-2]) * (y[-1] + y[-2]) + 0.5 * (x[1] - x[0]) * (y[1] + y[0])
+    result = basic(0, n - 3) + basic(1, n - 2)  # panels starting at 0 .. n-4 and at 1 .. n-3
+    return result / 2.0 + val / 2.0
 
 
 def time_grid(t0, tf, dt):
@@ -255,6 +315,133 @@ class SHOracle:
             if i == len(times) - 1:
                 break
             g = self.evolve(g, times[i + 1] - times[i], times[i + 1], nominal_time_step)
+        out = (g, np.array(norms), np.array(ips, dtype=np.complex128), np.array(lnorm))
+        return out + (np.array(nbl),) if by_l else out
+
+
+# ---------------------------------------------------------------------------
+# SphericalHarmonicMesh + SphericalHarmonicVelocityGaugeOperators + SplitInteractionOperator
+
+
+class SHVelocityOracle(SHOracle):
+    """
+    Velocity-gauge SphericalHarmonicMesh evolved by SplitInteractionOperator.
+
+    H0 and the step frame are the length gauge's (evolution_methods.py:89-123).
+    Interaction without the field: h1 = diags(-h1_off, h1_off) on the
+    l-flattening, h1_off = i hbar (q / m) c_l (l + 1) / r (0 across l-line
+    ends), and h2 = i hbar q / (2 m dr) kron(C, Alpha) on the r-flattening
+    (mesh_operators.py:1143-1180). Both are scaled by the vector potential
+    -simps(E, times_to_current) (mesh_operators.py:1182-1186, pulses.py:73-78).
+    Split factors: split_h1 (mesh_operators.py:1201-1250) and split_h2
+    (mesh_operators.py:1252-1408); the step applies h1 even, h1 odd, the four
+    h2 similarity factors, CN, and all of them reversed, then the mask.
+    """
+
+    def __init__(self, *, test_mass=u.electron_mass_reduced, **kw):
+        super().__init__(**kw)
+        self.test_mass = test_mass
+        self.h1, self.h2 = self._velocity_matrices()
+
+    def _velocity_matrices(self):
+        N, L, n = self.mesh_points, self.l_bound, self.r_points
+        qm = self.test_charge / self.test_mass
+        k = np.arange(N - 1)
+        l_of_k = k % L
+        h1_off = np.where((k + 1) % L != 0, self.c_l(l_of_k) * (l_of_k + 1), 0.0).astype(np.complex128)
+        h1_off *= 1j * u.hbar * qm / self.flatten(self.r_mesh, "l")[:-1]
+        h1 = sparse.diags((-h1_off, h1_off), offsets=(-1, 1))
+        alpha_vec = self.alpha(np.arange(n - 1, dtype=np.complex128))
+        alpha_block = sparse.diags((-alpha_vec, alpha_vec), offsets=(-1, 1))
+        c_vec = self.c_l(np.arange(L - 1, dtype=np.complex128))
+        c_block = sparse.diags((c_vec, c_vec), offsets=(-1, 1))
+        h2 = (1j * u.hbar * self.test_charge / (self.test_mass * 2 * self.delta_r)) * sparse.kron(
+            c_block, alpha_block, format="dia")
+        return h1, h2
+
+    def vector_potential(self, times_to_current):
+        return -simps_old(self.field(times_to_current), times_to_current)
+
+    @staticmethod
+    def _pair_rotation(a, n, parity, first_shift):
+        """(diag, off) of the rotations [[cos a_k, sin a_k], [-sin a_k, cos a_k]] on the
+        pairs (k, k + 1), k = first_shift + parity, first_shift + parity + 2, ... < n - 1;
+        elements outside every pair keep diag 1."""
+        diag = np.ones(n, dtype=np.complex128)
+        off = np.zeros(n - 1, dtype=np.complex128)
+        ks = np.arange(first_shift + parity, n - 1, 2)
+        diag[ks] = np.cos(a[ks])
+        diag[ks + 1] = np.cos(a[ks])
+        off[ks] = np.sin(a[ks])
+        return diag, off
+
+    def split_h1(self, apot, tau):
+        """split_h1 (mesh_operators.py:1201-1250): pairs (k, k + 1) of the l-flattening,
+        even k then odd k; angle a_k = -i tau A h1_off_k (real)."""
+        a = (tau * (-1j)) * (apot * self.h1).data[-1][1:]
+        N = self.mesh_points
+        ops = []
+        for parity in (0, 1):
+            diag, off = self._pair_rotation(np.concatenate([a, [0.0]]), N, parity, 0)
+            ops.append(_Op("dot", sparse.diags((-off, diag, off), offsets=(-1, 0, 1)), "l"))
+        return ops
+
+    def split_h2(self, apot, tau):
+        """split_h2 (mesh_operators.py:1252-1408): for the l-pairs (l, l + 1) of even l, then of
+        odd l, rotations on the radial pairs of even j, then of odd j, of the rows s = g_l + g_{l+1}
+        (angle +theta) and d = g_l - g_{l+1} (angle -theta), theta_j = -i tau A h2[(l, j), (l+1, j+1)]."""
+        L, n = self.l_bound, self.r_points
+        a = ((apot * self.h2).data[-1][n + 1:] * tau * (-1j))
+        a = np.concatenate([a, np.zeros(n + 1)])
+        ops = []
+        for l_par in (0, 1):
+            for r_par in (0, 1):
+                diag = np.ones(L * n, dtype=np.complex128)
+                off = np.zeros(L * n, dtype=np.complex128)
+                for l0 in range(l_par, L - 1, 2):
+                    theta = a[l0 * n:(l0 + 1) * n].copy()
+                    theta[n - 1:] = 0.0  # no pair (n - 1, n)
+                    for row, sign in ((l0, 1.0), (l0 + 1, -1.0)):
+                        d, o = self._pair_rotation(theta, n, r_par, 0)
+                        diag[row * n:(row + 1) * n] = d
+                        off[row * n:row * n + n - 1] = sign * o
+                off = off[:-1]
+                m = sparse.diags((-off, diag, off), offsets=(-1, 0, 1))
+                ops.append(_Op("sim", (m, "even" if l_par == 0 else "odd"), "r"))
+        return ops
+
+    def evolution_operators_velocity(self, time_step, apot):
+        tau = time_step / (2 * u.hbar)
+        cn = [
+            _Op("dot", add_to_diagonal((-1j * tau) * self.h0, value=1), "r"),
+            _Op("tdma", add_to_diagonal((1j * tau) * self.h0, value=1), "r"),
+        ]
+        split = (*self.split_h1(apot, tau), *self.split_h2(apot, tau))
+        return (*split, *cn, *reversed(split))
+
+    def evolve_velocity(self, g, time_step, times_to_current):
+        apot = self.vector_potential(times_to_current)
+        g = apply_sequentially(self.flatten, self.wrap, g, self.evolution_operators_velocity(time_step, apot))
+        if self.mask_mesh is not None:
+            g = g * self.mask_mesh
+        return g
+
+    def run(self, g0, times, nominal_time_step=None, store_every=1, basis=(), by_l=False):
+        """MeshSimulation.run with the velocity-gauge step; times_to_current = times[: i + 2]
+        when evolving from index i (sims.py:195-196, 317-321)."""
+        g = np.array(g0, dtype=np.complex128)
+        didx = set(data_indices(len(times), store_every).tolist())
+        norms, ips, lnorm, nbl = [], [], [], []
+        for i in range(len(times)):
+            if i in didx:
+                norms.append(self.norm(g))
+                ips.append([self.inner_product(phi, l, g) for (phi, l) in basis])
+                lnorm.append((np.sum(np.conj(g[-1]) * g[-1]) * self.delta_r).real)
+                if by_l:
+                    nbl.append(self.norm_by_l(g))
+            if i == len(times) - 1:
+                break
+            g = self.evolve_velocity(g, times[i + 1] - times[i], times[: i + 2])
         out = (g, np.array(norms), np.array(ips, dtype=np.complex128), np.array(lnorm))
         return out + (np.array(nbl),) if by_l else out
 

@@ -14,15 +14,16 @@ from oracle import mesh_cpu, physics
 
 def sh_case(*, r_bound_bohr=30.0, r_points=200, l_bound=20, pulse_width_as=100.0, fluence_jcm2=1.0,
             phase=0.0, t_half_as=300.0, dt_as=1.0, n_basis_l=3, store_every=1, mask=True, rows=None,
-            norm_by_l=False):
-    """Length-gauge SH + split operator with a sinc pulse, an absorber and a numeric 1s start."""
+            norm_by_l=False, gauge="length", layout=None):
+    """SH + split operator (length or velocity gauge) with a sinc pulse, an absorber and a numeric 1s start."""
     r_bound = r_bound_bohr * u.bohr_radius
     pw = pulse_width_as * u.asec
     F = fluence_jcm2 * u.Jcm2
     field = lambda t: physics.sinc_pulse_field(t, pulse_width=pw, fluence=F, phase=phase)
     maskf = (lambda r: physics.radial_cosine_mask(r, 0.8 * r_bound, r_bound)) if mask else None
-    orc = mesh_cpu.SHOracle(r_bound=r_bound, r_points=r_points, l_bound=l_bound, potential=physics.coulomb,
-                            field=field, mask=maskf)
+    oracle_type = mesh_cpu.SHVelocityOracle if gauge == "velocity" else mesh_cpu.SHOracle
+    orc = oracle_type(r_bound=r_bound, r_points=r_points, l_bound=l_bound, potential=physics.coulomb,
+                      field=field, mask=maskf)
     # numeric eigenstates of each low l block of H0 (dense, small); 1s is the l = 0 ground state
     h0 = orc.h0.tocsr()
     n = r_points
@@ -44,7 +45,7 @@ def sh_case(*, r_bound_bohr=30.0, r_points=200, l_bound=20, pulse_width_as=100.0
     times = mesh_cpu.time_grid(-t_half_as * u.asec, t_half_as * u.asec, dt)
     comp = engine.compile_sh(r_bound=r_bound, r_points=r_points, l_bound=l_bound, potential=physics.coulomb,
                              field=field, mask=maskf, times=times, time_step=dt, g0=g0, basis=basis,
-                             store_every=store_every, rows=rows, norm_by_l=norm_by_l)
+                             store_every=store_every, rows=rows, norm_by_l=norm_by_l, gauge=gauge, layout=layout)
     return dict(oracle=orc, compiled=comp, g0=g0, times=times, dt=dt, basis=basis, store_every=store_every)
 
 

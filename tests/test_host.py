@@ -251,3 +251,67 @@ def test_cyl_compile_tables_follow_reference_grid():
     assert np.allclose(c.roff[: nr - 1], rp * j / np.sqrt(j ** 2 - 0.25), rtol=1e-15, atol=0)
     assert not c.roff[nr - 1:].any()
     assert np.allclose(c.ts, np.diff(times) / (2 * u.hbar), rtol=1e-15, atol=0)
+
+
+# ------------------------------------------------------------------ velocity gauge (host side)
+
+
+def test_vector_potential_table_matches_oracle_quadrature():
+    """A_n = -simps(E, times[: n + 2]) per step (mesh_operators.py:1182-1186), from the O(n)
+    running sums, equals the oracle's per-call restatement of old SciPy simps."""
+    fld = lambda t: physics.sinc_pulse_field(t, pulse_width=80 * u.asec, fluence=0.5 * u.Jcm2, phase=0.4)
+    times = mesh_cpu.time_grid(-40 * u.asec, 40 * u.asec, 1 * u.asec)
+    table = engine.vector_potential_table(fld, times)
+    assert len(table) == len(times) - 1
+    ref = np.array([-mesh_cpu.simps_old(fld(times[: n + 2]), times[: n + 2]) for n in range(len(times) - 1)])
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(table - ref)) <= 1e-13 * scale
+
+
+def test_fields_datastore_vector_potential_uses_the_same_rule():
+    sim = _small_sh_spec().to_sim()
+    fld = sim.spec.electric_potential.get_electric_field_amplitude
+    ref = np.array([-mesh_cpu.simps_old(fld(sim.times[: i + 1]), sim.times[: i + 1]) for i in sim.data_indices])
+    got = sim.datastores_by_type[mesh.Fields].vector_potential_amplitude
+    assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+    t = sim.times[:30]
+    assert np.isclose(sim.spec.electric_potential.get_vector_potential_amplitude_numeric(t),
+                      -mesh_cpu.simps_old(fld(t), t), rtol=1e-13)
+
+
+def test_velocity_tables_reproduce_oracle_couplings():
+    """The velocity rows and the field table encode the oracle's h1 / h2 entries and A(t)."""
+    sim = _small_sh_spec(operators=mesh.SphericalHarmonicVelocityGaugeOperators()).to_sim()
+    comp = sim.spec.evolution_method.compile(sim)
+    assert comp.gauge == "velocity" and comp.tables.shape[0] == engine.SH_NTAB + 1
+    n, L = 150, 12
+    tau0 = sim.spec.time_step / (2 * u.hbar)
+    orc = mesh_cpu.SHVelocityOracle(r_bound=sim.spec.r_bound, r_points=n, l_bound=L, potential=physics.coulomb,
+                                    field=lambda t: 0.0 * t, test_mass=sim.spec.test_mass)
+    tab = engine.from_positions(comp.tables, n, comp.P, comp.m)
+    RQ, AV = tab[engine.SH_RQ], tab[engine.SH_NTAB]
+    h1, h2 = orc.h1.tocsr(), orc.h2.tocsr()
+    for j in (0, 7, 149):
+        for l in (0, 3, 10):
+            k = j * L + l  # l-flattening
+            cl = (l + 1) / np.sqrt((2 * l + 1) * (2 * l + 3))
+            assert np.isclose(RQ[j] * cl * (l + 1) / tau0, h1[k, k + 1].imag, rtol=1e-13)
+            if j < n - 1:
+                kr = l * n + j  # r-flattening: (l, j) -> (l + 1, j + 1)
+                assert np.isclose(AV[j] * cl / tau0, h2[kr, kr + n + 1].imag, rtol=1e-13)
+    assert AV[n - 1] == 0.0
+    fld = sim.spec.electric_potential.get_electric_field_amplitude
+    a = np.array([-mesh_cpu.simps_old(fld(sim.times[: i + 2]), sim.times[: i + 2]) for i in range(comp.n_steps)])
+    assert np.allclose(comp.fs, a * np.diff(sim.times) / sim.spec.time_step, rtol=1e-12, atol=1e-15 * np.abs(a).max())
+
+
+def test_velocity_gauge_descriptor_and_limits():
+    assert "gauge" in [f[0] for f in _lib.ShDesc._fields_]
+    with pytest.raises(exceptions.DeviceError):
+        engine.compile_sh(r_bound=10 * u.bohr_radius, r_points=2, l_bound=4, potential=physics.coulomb,
+                          field=lambda t: 0.0 * t, mask=None, times=np.arange(3) * u.asec, time_step=u.asec,
+                          g0=np.zeros((4, 2)), gauge="velocity")
+    with pytest.raises(exceptions.DeviceError):
+        engine.compile_sh(r_bound=10 * u.bohr_radius, r_points=20, l_bound=4, potential=physics.coulomb,
+                          field=lambda t: 0.0 * t, mask=None, times=np.arange(3) * u.asec, time_step=u.asec,
+                          g0=np.zeros((4, 20)), gauge="magnetic")

@@ -356,3 +356,142 @@ def test_arpack_and_lapack_eigenbases_agree():
         assert abs(abs(np.vdot(v1[:, i], v2[:, i])) - 1) < 1e-8
     # hydrogen 2p within discretisation error of -Ry/4
     assert abs(w2[0] / (-u.rydberg / 4) - 1) < 5e-3
+
+
+# ------------------------------------------------------------------ velocity gauge
+
+
+def _tiny_vel(r_points, l_bound, **kw):
+    args = dict(r_bound=10 * u.bohr_radius, r_points=r_points, l_bound=l_bound, potential=physics.coulomb,
+                field=lambda t: 0.02 * u.atomic_electric_field * np.cos(t / (50 * u.asec)))
+    args.update(kw)
+    return mesh_cpu.SHVelocityOracle(**args)
+
+
+def _masked_part(H, keep):
+    out = np.zeros_like(H)
+    rows, cols = np.nonzero(H)
+    for a, b in zip(rows, cols):
+        if keep(a, b):
+            out[a, b] = H[a, b]
+    return out
+
+
+@pytest.mark.parametrize("r_points, l_bound", [(7, 5), (6, 4), (5, 6), (4, 3), (3, 2)])
+def test_velocity_split_factors_are_exponentials_of_their_couplings(r_points, l_bound):
+    """
+    Each velocity-gauge split factor is exp(-i tau part) of its part of the interaction,
+    the parts built from the reference's definitions (mesh_operators.py:1143-1180) as dense
+    matrices: h1 (l-flattening) split by the parity of the pair's first index; h2 =
+    i hbar q / (2 m dr) kron(C, Alpha) (r-flattening) split by the parities of the lower l
+    and the lower j of each coupled pair. The similarity transform, the sign of the d row,
+    the angles and the pairing are all pinned by this (split_h1 / split_h2,
+    mesh_operators.py:1201-1408, SimilarityOperator :150-204).
+    """
+    L, n = l_bound, r_points
+    o = _tiny_vel(n, L)
+    rng = np.random.default_rng(n * 10 + L)
+    g = rng.standard_normal((L, n)) + 1j * rng.standard_normal((L, n))
+    tau = 3 * u.asec / (2 * u.hbar)
+    A = 0.7 * u.atomic_electric_field * u.atomic_time
+    # dense h2 from the definition, independently of the oracle's sparse build
+    cvec = (np.arange(L - 1) + 1) / np.sqrt((2 * np.arange(L - 1) + 1) * (2 * np.arange(L - 1) + 3))
+    jj = np.arange(n - 1)
+    avec = (jj ** 2 + 2 * jj + 1) / (jj ** 2 + 2 * jj + 0.75)
+    C = np.diag(cvec, 1) + np.diag(cvec, -1)
+    Al = np.diag(avec, 1) - np.diag(avec, -1)
+    H2 = A * 1j * u.hbar * u.electron_charge / (u.electron_mass_reduced * 2 * o.delta_r) * np.kron(C, Al)
+    assert np.allclose((A * o.h2).toarray(), H2, rtol=1e-14, atol=0)
+    for op, (lp, rp) in zip(o.split_h2(A, tau), [(0, 0), (0, 1), (1, 0), (1, 1)]):
+        part = _masked_part(H2, lambda a, b: min(a // n, b // n) % 2 == lp and min(a % n, b % n) % 2 == rp)
+        ref = (sl.expm(-1j * tau * part) @ g.reshape(-1)).reshape(L, n)
+        got = mesh_cpu.apply_sequentially(o.flatten, o.wrap, g, [op])
+        assert rel_l2(got, ref) < 1e-13
+    # dense h1: (k, k+1) of the l-flattening k = j L + l, i hbar (q / m) c_l (l + 1) / r_j above the diagonal
+    H1 = np.zeros((L * n, L * n), dtype=np.complex128)
+    for k in range(L * n - 1):
+        l, j = k % L, k // L
+        if (k + 1) % L:
+            v = A * 1j * u.hbar * (u.electron_charge / u.electron_mass_reduced) * cvec[l] * (l + 1) / o.r[j]
+            H1[k, k + 1], H1[k + 1, k] = v, -v
+    for op, par in zip(o.split_h1(A, tau), (0, 1)):
+        part = _masked_part(H1, lambda a, b: min(a, b) % 2 == par)
+        ref = sl.expm(-1j * tau * part) @ g.reshape(-1, order="F")
+        got = mesh_cpu.apply_sequentially(o.flatten, o.wrap, g, [op])
+        assert rel_l2(got.reshape(-1, order="F"), ref) < 1e-13
+
+
+def test_velocity_split_product_is_third_order():
+    """The symmetric product of the six factors approximates exp(-2 i tau (h1 + h2) A) with a
+    local error that falls 8x per halving of tau (a dropped factor or a wrong sign would not)."""
+    L, n = 5, 7
+    o = _tiny_vel(n, L)
+    rng = np.random.default_rng(2)
+    g = rng.standard_normal((L, n)) + 1j * rng.standard_normal((L, n))
+    A = 0.7 * u.atomic_electric_field * u.atomic_time
+    Q = np.zeros((L * n, L * n))  # r-order index l n + j from l-order index j L + l
+    for l in range(L):
+        for j in range(n):
+            Q[l * n + j, j * L + l] = 1
+    H = (A * o.h2).toarray() + Q @ (A * o.h1).toarray() @ Q.T
+    errs = []
+    for tau in np.array([3.0, 1.5, 0.75]) * u.asec / (2 * u.hbar):
+        split = (*o.split_h1(A, tau), *o.split_h2(A, tau))
+        got = mesh_cpu.apply_sequentially(o.flatten, o.wrap, g, (*split, *reversed(split)))
+        ref = (sl.expm(-2j * tau * H) @ g.reshape(-1)).reshape(L, n)
+        errs.append(np.abs(got - ref).max())
+    assert 7.0 < errs[0] / errs[1] < 9.0 and 7.0 < errs[1] / errs[2] < 9.0
+
+
+def test_velocity_gauge_without_field_is_the_length_gauge_step():
+    """A = 0: every split factor is the identity, so both gauges take the same CN step."""
+    kw = dict(r_bound=10 * u.bohr_radius, r_points=9, l_bound=4, potential=physics.coulomb, field=lambda t: 0.0 * t,
+              mask=lambda r: physics.radial_cosine_mask(r, 6 * u.bohr_radius, 10 * u.bohr_radius))
+    rng = np.random.default_rng(3)
+    g = rng.standard_normal((4, 9)) + 1j * rng.standard_normal((4, 9))
+    times = np.arange(5) * 2 * u.asec
+    gv = mesh_cpu.SHVelocityOracle(**kw).run(g, times)[0]
+    gl = mesh_cpu.SHOracle(**kw).run(g, times, 2 * u.asec)[0]
+    assert rel_l2(gv, gl) < 1e-14
+
+
+def test_velocity_gauge_evolution_without_mask_is_unitary():
+    fld = lambda t: physics.sinc_pulse_field(t, pulse_width=50 * u.asec, fluence=1 * u.Jcm2)
+    o = mesh_cpu.SHVelocityOracle(r_bound=20 * u.bohr_radius, r_points=60, l_bound=8, potential=physics.coulomb,
+                                  field=fld)
+    rng = np.random.default_rng(5)
+    g = rng.standard_normal((8, 60)) + 1j * rng.standard_normal((8, 60))
+    times = mesh_cpu.time_grid(-40 * u.asec, 40 * u.asec, 1 * u.asec)
+    gf, norms, _, _ = o.run(g, times)
+    assert abs(norms[-1] - norms[0]) < 1e-12 * norms[0]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 11, 30])
+def test_simps_old_matches_scipy_simpson_on_odd_counts_and_avg_rule_on_even(n):
+    """
+    Odd sample counts: SciPy's composite Simpson (the rule is the same in every SciPy
+    release, so the local scipy.integrate.simpson pins it). Even counts: the mean of the
+    two Simpson + trapezoid splits, each Simpson part from scipy.integrate.simpson on an
+    odd count. Parity with the reference's SciPy (< 1.11, even='avg') rests on this rule.
+    """
+    rng = np.random.default_rng(n)
+    x = np.sort(rng.uniform(-3, 4, n)) if n > 1 else np.zeros(1)
+    y = np.sin(x) + 0.1 * x ** 3
+    got = mesh_cpu.simps_old(y, x)
+    if n == 1:
+        assert got == 0.0
+        return
+    if n % 2:
+        assert np.isclose(got, integrate.simpson(y, x=x), rtol=1e-13, atol=1e-15)
+        return
+    simp = lambda a, b: integrate.simpson(y[a:b], x=x[a:b]) if b - a >= 3 else 0.0
+    trap = lambda a: 0.5 * (x[a + 1] - x[a]) * (y[a + 1] + y[a])
+    ref = 0.5 * ((simp(0, n - 1) + trap(n - 2)) + (trap(0) + simp(1, n)))
+    assert np.isclose(got, ref, rtol=1e-13, atol=1e-15)
+
+
+def test_simps_old_exact_for_cubics_on_uniform_odd_grids():
+    x = np.linspace(-1.0, 2.0, 9)
+    y = 2 * x ** 3 - x ** 2 + 3
+    exact = (2 * 2 ** 4 / 4 - 2 ** 3 / 3 + 6) - (2 / 4 + 1 / 3 - 3)
+    assert np.isclose(mesh_cpu.simps_old(y, x), exact, rtol=1e-14)
