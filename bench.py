@@ -71,10 +71,46 @@ def time_batched(compiled, steps, warmup):
     ctx.profile(True)
     ctx.run(steps, use_graph=False)
     prof = ctx.profile_read()
+    prof["vphase_ms"], prof["vphase_launches"] = ctx.profile_kernel(4)
     ctx.profile(False)
     used, total = ctx.pool_usage()
     ctx.close()
     return dict(points=points, wall_s=wall, dev_ms=dev_ms, prof=prof, pool_used=used)
+
+
+def time_velocity(params, store_every, steps, warmup):
+    """
+    The same scan draws in the velocity gauge (SphericalHarmonicVelocityGaugeOperators +
+    SplitInteractionOperator): per step the L-phase (h1 rotations, mask), the h2
+    similarity factors of the even and the odd l-pairs, the R-phase, and the h2 factors
+    again in reverse: six read+write sweeps of g (192 B per mesh point and step).
+    """
+    from ionization_amd import mesh, workloads
+
+    t = time.perf_counter()
+    compiled = []
+    for p in params:
+        sim = workloads.sh_scan_spec(p, store_data_every=store_every,
+                                     operators=mesh.SphericalHarmonicVelocityGaugeOperators()).to_sim()
+        compiled.append(sim.spec.evolution_method.compile(sim))
+    build_s = time.perf_counter() - t
+    b = time_batched(compiled, steps, warmup)
+    rate = b["points"] * steps / b["wall_s"]
+    prof = b["prof"]
+    # h2 sweeps read and write the rows inside l-pairs (of each parity), before and after the solve
+    h2_bytes = sum(2 * 32 * c.r_points * (2 * (c.l_bound // 2) + 2 * ((c.l_bound - 1) // 2)) for c in compiled)
+    h2_ms = prof["vphase_ms"] / steps
+    out = {
+        "workload": f"sh_scan draws ({len(compiled)} simulations), velocity gauge, one hipGraph replay per step",
+        "value": rate, "unit": "mesh-point-steps/s", "ms_per_step": b["wall_s"] * 1e3 / steps,
+        "bytes_per_point_step": 192, "step_hbm_gbs": rate * 192 / 1e9, "build_s": build_s,
+        "kernels_ms_per_step": {"k_sh_lphase": prof["lphase_ms"] / steps, "k_sh_rphase": prof["rphase_ms"] / steps,
+                                "k_sh_vphase": h2_ms},
+        "k_sh_vphase_gbs": h2_bytes / (h2_ms * 1e-3) / 1e9 if h2_ms > 0 else None,
+    }
+    log(f"velocity gauge: {out['ms_per_step']:.3f} ms/step, {rate / 1e9:.2f} G mesh-point-steps/s; "
+        f"h2 factors {h2_ms:.3f} ms/step at {out['k_sh_vphase_gbs'] or 0:.0f} GB/s")
+    return out
 
 
 def time_plain(steps, warmup):
@@ -409,6 +445,7 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="also measure every candidate SH layout per radial size class and time the scan "
                          "under the measured layouts (reported as 'autotuned')")
+    ap.add_argument("--velocity", type=int, default=1, help="0 skips the velocity-gauge scan")
     ap.add_argument("--cyl-sims", type=int, default=32, help="0 skips the cylindrical ADI workload")
     ap.add_argument("--cyl-steps", type=int, default=100)
     ap.add_argument("--line-batch", type=int, default=256, help="0 skips the LineMesh workload")
@@ -485,6 +522,7 @@ def main():
     traffic = None if measured is None else measured[0] + measured[1]
     step_gbs = rate * BYTES_PER_POINT_STEP / 1e9
 
+    vel = time_velocity(params, args.store_every, args.steps, args.warmup) if args.velocity else None
     cyl = time_cyl(args.cyl_sims, args.seed, args.cyl_steps, 10) if args.cyl_sims > 0 else None
     line = time_line(args.line_batch, min(args.cpu_seconds, 5.0)) if args.line_batch > 0 else None
     served = (time_serve(args.serve_requests, args.serve_rate, args.seed, args.serve_pool_gb)
@@ -538,6 +576,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "cpu_baseline_parallel": cpu_par,
+        "velocity_gauge": vel,
         "cyl_adi": cyl,
         "line_cn": line,
         "served_stream": served,

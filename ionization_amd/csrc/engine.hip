@@ -139,7 +139,15 @@ struct Bucket {
     bool imag = false;  // Line: complex time steps (imaginary-time relaxation)
     int lanes = 1;      // cylindrical rho-sweep: lines per workgroup
     DevVec<int2> lines;  // SH: (slot, l); Line: (slot, 0); Cyl z: (slot, rho); Cyl rho: (slot, position group)
+    DevVec<int2> vpairs[2];  // SH: the velocity-gauge l-pairs (slot, l) of even / odd l among these simulations
 };
+
+void free_bucket(Bucket& b) {
+    if (b.lines.ptr) (void)hipFree(b.lines.ptr);
+    for (auto& v : b.vpairs)
+        if (v.ptr) (void)hipFree(v.ptr);
+    b = Bucket{};
+}
 
 // l segments of at most kSegRows rows per L-phase wave, balanced in size.
 constexpr int kSegRows = kShSegRows;
@@ -170,7 +178,6 @@ struct tdse_ctx {
     double* d_clv = nullptr;  // c_l (l + 1): the velocity gauge's h1 coupling per l
     DevVec<ShWave> waves;     // length-gauge simulations' waves first, then velocity-gauge ones
     int n_len_waves = 0;
-    std::vector<Bucket> sh_vbuckets[2];  // velocity-gauge l-pairs (slot, l) of even / odd l, by (rows, chunks)
     DevVec<int2> jobs;
     std::vector<Bucket> sh_buckets;
     std::vector<int> sh_order;  // sh_buckets by decreasing work
@@ -190,9 +197,10 @@ struct tdse_ctx {
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
-    std::vector<std::pair<int, int>> ev_pairs;  // (kind, start event index); kinds: SH L-phase, SH R-phase, Cyl z, Cyl rho
-    double prof_ms[4] = {0.0, 0.0, 0.0, 0.0};
-    int64_t prof_n[4] = {0, 0, 0, 0};
+    // (kind, start event index); kinds: SH L-phase, SH R-phase, Cyl z, Cyl rho, SH velocity-gauge h2 factors
+    std::vector<std::pair<int, int>> ev_pairs;
+    double prof_ms[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    int64_t prof_n[5] = {0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -292,8 +300,7 @@ int rebuild_tables(tdse_ctx* ctx) {
     if ((rc = devvec_upload(ctx, ctx->jobs, jobs))) return rc;
     if ((rc = devvec_upload(ctx, ctx->cyl_jobs, cjobs))) return rc;
     auto fill = [&](std::vector<Bucket>& dst, std::map<std::pair<int, int>, std::vector<int2>>& src) -> int {
-        for (auto& b : dst)
-            if (b.lines.ptr) (void)hipFree(b.lines.ptr);
+        for (auto& b : dst) free_bucket(b);
         dst.clear();
         for (auto& kv : src) {
             Bucket b;
@@ -307,15 +314,23 @@ int rebuild_tables(tdse_ctx* ctx) {
         return TDSE_OK;
     };
     if ((rc = fill(ctx->sh_buckets, shb))) return rc;
-    if ((rc = fill(ctx->sh_vbuckets[0], vpb[0]))) return rc;
-    if ((rc = fill(ctx->sh_vbuckets[1], vpb[1]))) return rc;
+    for (auto& b : ctx->sh_buckets)
+        for (int par = 0; par < 2; ++par) {
+            auto it = vpb[par].find({b.M, b.P});
+            if (it != vpb[par].end() && (rc = devvec_upload(ctx, b.vpairs[par], it->second))) return rc;
+        }
     if ((rc = fill(ctx->line_buckets, lnb))) return rc;
     if ((rc = fill(ctx->cyl_z_buckets, czb))) return rc;
     if ((rc = fill(ctx->cyl_r_buckets, crb))) return rc;
     {
         std::vector<std::pair<double, int>> w;
-        for (size_t i = 0; i < ctx->sh_buckets.size(); ++i)
-            w.push_back({-(double)ctx->sh_buckets[i].lines.count * ctx->sh_buckets[i].P * ctx->sh_buckets[i].M, (int)i});
+        for (size_t i = 0; i < ctx->sh_buckets.size(); ++i) {
+            // rows swept per step: the R-phase's lines, and the velocity-gauge h2 factors' rows
+            // (two rows per pair, before and after the solve)
+            const Bucket& b = ctx->sh_buckets[i];
+            const double rows = (double)b.lines.count + 4.0 * (double)(b.vpairs[0].count + b.vpairs[1].count);
+            w.push_back({-rows * b.P * b.M, (int)i});
+        }
         std::sort(w.begin(), w.end());
         ctx->sh_order.clear();
         for (auto& pr : w) ctx->sh_order.push_back(pr.second);
@@ -393,10 +408,11 @@ void dispatch_rphase(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
 }
 
 template <bool REV>
-void dispatch_vphase(tdse_ctx* ctx, const Bucket& b) {
-    const dim3 grid((unsigned)b.lines.count), block(b.P);
+void dispatch_vphase(tdse_ctx* ctx, const Bucket& b, int par, hipStream_t st) {
+    const DevVec<int2>& pairs = b.vpairs[par];
+    const dim3 grid((unsigned)pairs.count), block(b.P);
     auto args = [&](auto kernel) {
-        hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, ctx->d_sh, b.lines.ptr, (int)b.lines.count, ctx->d_cl);
+        hipLaunchKernelGGL(kernel, grid, block, 0, st, ctx->d_sh, pairs.ptr, (int)pairs.count, ctx->d_cl);
     };
     switch (b.M) {
         case 2: args(k_sh_vphase<2, REV>); break;
@@ -509,15 +525,28 @@ int launch_step(tdse_ctx* ctx, bool cyl_pre) {
         prof_end(ctx);
         const int nj = (int)ctx->jobs.count;
         hipLaunchKernelGGL(k_sh_observe, dim3(nj), dim3(256), 0, ctx->stream, ctx->d_sh, ctx->jobs.ptr, nj);
-        // velocity gauge, before the CN solve: h2 factors of the even l-pairs, then of the odd ones
-        for (int par = 0; par < 2; ++par)
-            for (auto& b : ctx->sh_vbuckets[par]) dispatch_vphase<false>(ctx, b);
+        // One bucket's chain: velocity-gauge h2 factors of its even, then odd l-pairs
+        // (before the CN solve), the R-phase, the factors in reverse (after it). The
+        // chains of different buckets touch different simulations.
+        auto chain = [&](const Bucket& b, hipStream_t st) {
+            for (int par = 0; par < 2; ++par)
+                if (b.vpairs[par].count) {
+                    prof_begin(ctx, 4);
+                    dispatch_vphase<false>(ctx, b, par, st);
+                    prof_end(ctx);
+                }
+            prof_begin(ctx, 1);
+            dispatch_rphase(ctx, b, st);
+            prof_end(ctx);
+            for (int par = 1; par >= 0; --par)
+                if (b.vpairs[par].count) {
+                    prof_begin(ctx, 4);
+                    dispatch_vphase<true>(ctx, b, par, st);
+                    prof_end(ctx);
+                }
+        };
         if (ctx->profiling || ctx->sh_buckets.size() < 2) {
-            for (auto& b : ctx->sh_buckets) {
-                prof_begin(ctx, 1);
-                dispatch_rphase(ctx, b, ctx->stream);
-                prof_end(ctx);
-            }
+            for (auto& b : ctx->sh_buckets) chain(b, ctx->stream);
         } else {
             // independent buckets run concurrently (largest first, round robin
             // over the context stream and the side streams) so that small
@@ -526,16 +555,13 @@ int launch_step(tdse_ctx* ctx, bool cyl_pre) {
             for (int j = 0; j < kSideStreams; ++j) HIP_TRY(ctx, hipStreamWaitEvent(ctx->side[j], ctx->ev_fork, 0));
             for (size_t i = 0; i < ctx->sh_order.size(); ++i) {
                 const int j = (int)(i % (kSideStreams + 1));
-                dispatch_rphase(ctx, ctx->sh_buckets[ctx->sh_order[i]], j == 0 ? ctx->stream : ctx->side[j - 1]);
+                chain(ctx->sh_buckets[ctx->sh_order[i]], j == 0 ? ctx->stream : ctx->side[j - 1]);
             }
             for (int j = 0; j < kSideStreams; ++j) {
                 HIP_TRY(ctx, hipEventRecord(ctx->ev_join[j], ctx->side[j]));
                 HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join[j], 0));
             }
         }
-        // velocity gauge, after the CN solve: the same factors in reverse order
-        for (int par = 1; par >= 0; --par)
-            for (auto& b : ctx->sh_vbuckets[par]) dispatch_vphase<true>(ctx, b);
         const int ns = (int)ctx->sh.size();
         hipLaunchKernelGGL(k_sh_advance, dim3((ns + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh, ns);
     }
@@ -681,17 +707,8 @@ int tdse_ctx_destroy(tdse_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     drop_graph(ctx);
-    for (auto& b : ctx->sh_buckets)
-        if (b.lines.ptr) (void)hipFree(b.lines.ptr);
-    for (int par = 0; par < 2; ++par)
-        for (auto& b : ctx->sh_vbuckets[par])
-            if (b.lines.ptr) (void)hipFree(b.lines.ptr);
-    for (auto& b : ctx->line_buckets)
-        if (b.lines.ptr) (void)hipFree(b.lines.ptr);
-    for (auto& b : ctx->cyl_z_buckets)
-        if (b.lines.ptr) (void)hipFree(b.lines.ptr);
-    for (auto& b : ctx->cyl_r_buckets)
-        if (b.lines.ptr) (void)hipFree(b.lines.ptr);
+    for (auto* v : {&ctx->sh_buckets, &ctx->line_buckets, &ctx->cyl_z_buckets, &ctx->cyl_r_buckets})
+        for (auto& b : *v) free_bucket(b);
     if (ctx->cyl_jobs.ptr) (void)hipFree(ctx->cyl_jobs.ptr);
     if (ctx->d_cyl) (void)hipFree(ctx->d_cyl);
     if (ctx->waves.ptr) (void)hipFree(ctx->waves.ptr);
@@ -1384,7 +1401,7 @@ int tdse_last_run_ms(tdse_ctx* ctx, double* ms) {
 int tdse_profile(tdse_ctx* ctx, int32_t enable) {
     if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
     ctx->profiling = enable != 0;
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 5; ++i) {
         ctx->prof_ms[i] = 0.0;
         ctx->prof_n[i] = 0;
     }
@@ -1403,7 +1420,7 @@ int tdse_profile_read(tdse_ctx* ctx, double* lphase_ms, int64_t* lphase_launches
 
 int tdse_profile_kernel(tdse_ctx* ctx, int32_t kernel, double* ms, int64_t* launches) {
     if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
-    if (kernel < 0 || kernel > 3) return fail(ctx, TDSE_ERR_ARG, "kernel must be 0..3");
+    if (kernel < 0 || kernel > 4) return fail(ctx, TDSE_ERR_ARG, "kernel must be 0..4");
     if (ms) *ms = ctx->prof_ms[kernel];
     if (launches) *launches = ctx->prof_n[kernel];
     return TDSE_OK;

@@ -780,7 +780,8 @@ class DeviceContext:
                 "rphase_launches": rn.value}
 
     def profile_kernel(self, kernel: int):
-        """(ms, launches) accumulated for kernel 0 = SH L-phase, 1 = SH R-phase, 2 = Cyl z, 3 = Cyl rho."""
+        """(ms, launches) accumulated for kernel 0 = SH L-phase, 1 = SH R-phase, 2 = Cyl z, 3 = Cyl rho,
+        4 = SH velocity-gauge h2 factors."""
         ms, n = ctypes.c_double(), ctypes.c_int64()
         self._check(self._lib.tdse_profile_kernel(self._ctx, int(kernel), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value

@@ -56,11 +56,11 @@ def sh_scan_params(n=256, seed=0):
     return out
 
 
-def sh_scan_spec(p, store_data_every=100):
+def sh_scan_spec(p, store_data_every=100, **kwargs):
     return sh_single_spec(
         f"scan_{p['index']:04d}", r_bound=p["r_bound"], r_points=p["r_points"], l_bound=p["l_bound"],
         pulse_width=p["pulse_width"], fluence=p["fluence"], phase=p["phase"], store_data_every=store_data_every,
-        numeric_eigenstate_max_energy=0.0)
+        numeric_eigenstate_max_energy=0.0, **kwargs)
 
 
 def line_ground_state(*, z_bound, z_points, potential, test_mass, imaginary_dt=0.5 * u.atomic_time, n_iter=400,
