@@ -120,10 +120,15 @@ int tdse_sh_add(tdse_ctx *ctx, const tdse_sh_desc *desc, const double *tables, c
 
 /*
  * LineMesh simulation with the length-gauge Crank-Nicolson step
- * (meshes.py:263-325, mesh_operators.py:304-327, evolution_methods.py:46-77).
+ * (meshes.py:263-325, mesh_operators.py:304-327, evolution_methods.py:46-77),
+ * or with SplitInteractionOperator (method 1: length gauge, the phase
+ * exp(-i tau V_int) around a CN step in K + V, mesh_operators.py:329-342;
+ * method 2: velocity gauge, z-pair rotations around it, mesh_operators.py:
+ * 352-427; evolution_methods.py:89-123).
  * tables: 5 * npad doubles (H0 diagonal tau0-scaled, off-diagonal to j-1,
  * off-diagonal to j+1, tau0 (-q) z, mask); wt: N complex (i tau_n / tau0);
- * fs: N doubles (field at t_{n+1}); basis: n_basis * npad complex.
+ * fs: N doubles (field at t_{n+1}; method 2: the vector potential
+ * -integral of E up to t_{n+1}); basis: n_basis * npad complex.
  */
 typedef struct {
     int32_t z_points;      /* n */
@@ -133,11 +138,13 @@ typedef struct {
     int32_t store_every;
     int32_t n_basis;
     int32_t renormalize;   /* 1: renormalise after every step (imaginary-time relaxation) */
-    int32_t reserved;
+    int32_t method;        /* 0: ADI (CN in the total H); 1, 2: split operator, length / velocity gauge
+                              (real time steps, no renormalisation) */
     double delta_z;
     double offdiag;        /* tau0 P_z, the off-diagonal inside the line (tables 1 and 2 hold it too) */
     double zq_first;       /* tau0 (-q) z_0: the z coupling of row j is zq_first + j zq_step */
     double zq_step;
+    double vel_factor;     /* method 2: tau0 hbar (q / m) / (2 dz), the z-pair angle per unit fs * tau_n / tau0 */
 } tdse_line_desc;
 
 int tdse_line_add(tdse_ctx *ctx, const tdse_line_desc *desc, const double *tables, const double *wt,

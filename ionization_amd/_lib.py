@@ -46,11 +46,12 @@ class LineDesc(ctypes.Structure):
         ("store_every", ctypes.c_int32),
         ("n_basis", ctypes.c_int32),
         ("renormalize", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("method", ctypes.c_int32),
         ("delta_z", ctypes.c_double),
         ("offdiag", ctypes.c_double),
         ("zq_first", ctypes.c_double),
         ("zq_step", ctypes.c_double),
+        ("vel_factor", ctypes.c_double),
     ]
 
 

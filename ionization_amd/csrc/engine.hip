@@ -137,6 +137,7 @@ struct Bucket {
     int M = 0;  // exact rows per chunk-thread
     int P = 0;
     bool imag = false;  // Line: complex time steps (imaginary-time relaxation)
+    int method = 0;     // Line: LINE_ADI, LINE_SIO_LEN or LINE_SIO_VEL
     int lanes = 1;      // cylindrical rho-sweep: lines per workgroup
     DevVec<int2> lines;  // SH: (slot, l); Line: (slot, 0); Cyl z: (slot, rho); Cyl rho: (slot, position group)
     DevVec<int2> vpairs[2];  // SH: the velocity-gauge l-pairs (slot, l) of even / odd l among these simulations
@@ -279,7 +280,7 @@ int rebuild_tables(tdse_ctx* ctx) {
     for (int s = 0; s < (int)ctx->line.size(); ++s) {
         const LineHost& h = ctx->line[s];
         if (!h.live) continue;
-        lnb[{h.dev.m, h.dev.P + (h.complex_steps ? 4096 : 0)}].push_back(make_int2(s, 0));
+        lnb[{h.dev.m, h.dev.P + (h.complex_steps ? 4096 : 0) + 8192 * h.dev.method}].push_back(make_int2(s, 0));
     }
     std::vector<int2> cjobs;
     std::map<std::pair<int, int>, std::vector<int2>> czb;             // (rows, chunks)
@@ -306,7 +307,8 @@ int rebuild_tables(tdse_ctx* ctx) {
             Bucket b;
             b.M = kv.first.first;
             b.P = kv.first.second % 4096;
-            b.imag = kv.first.second >= 4096;
+            b.imag = (kv.first.second % 8192) >= 4096;
+            b.method = kv.first.second / 8192;
             int r = devvec_upload(ctx, b.lines, kv.second);
             if (r) return r;
             dst.push_back(b);
@@ -426,30 +428,34 @@ void dispatch_vphase(tdse_ctx* ctx, const Bucket& b, int par, hipStream_t st) {
     }
 }
 
-template <int M, bool IMAG>
+template <int M, bool IMAG, int METHOD>
 void launch_line(tdse_ctx* ctx, const Bucket& b, int n_steps) {
     const size_t lds = 3 * (size_t)b.P * sizeof(double2) + 64 * sizeof(double);
-    hipLaunchKernelGGL((k_line_steps<M, 512, IMAG>), dim3((unsigned)b.lines.count), dim3(b.P), lds, ctx->stream,
-                       ctx->d_line, b.lines.ptr, (int)b.lines.count, n_steps);
+    hipLaunchKernelGGL((k_line_steps<M, 512, IMAG, METHOD>), dim3((unsigned)b.lines.count), dim3(b.P), lds,
+                       ctx->stream, ctx->d_line, b.lines.ptr, (int)b.lines.count, n_steps);
 }
 
-template <bool IMAG>
+template <bool IMAG, int METHOD>
 void dispatch_line_m(tdse_ctx* ctx, const Bucket& b, int n_steps) {
     switch (b.M) {
-        case 2: launch_line<2, IMAG>(ctx, b, n_steps); break;
-        case 4: launch_line<4, IMAG>(ctx, b, n_steps); break;
-        case 6: launch_line<6, IMAG>(ctx, b, n_steps); break;
-        case 8: launch_line<8, IMAG>(ctx, b, n_steps); break;
-        case 10: launch_line<10, IMAG>(ctx, b, n_steps); break;
-        case 12: launch_line<12, IMAG>(ctx, b, n_steps); break;
-        case 14: launch_line<14, IMAG>(ctx, b, n_steps); break;
-        default: launch_line<16, IMAG>(ctx, b, n_steps); break;
+        case 2: launch_line<2, IMAG, METHOD>(ctx, b, n_steps); break;
+        case 4: launch_line<4, IMAG, METHOD>(ctx, b, n_steps); break;
+        case 6: launch_line<6, IMAG, METHOD>(ctx, b, n_steps); break;
+        case 8: launch_line<8, IMAG, METHOD>(ctx, b, n_steps); break;
+        case 10: launch_line<10, IMAG, METHOD>(ctx, b, n_steps); break;
+        case 12: launch_line<12, IMAG, METHOD>(ctx, b, n_steps); break;
+        case 14: launch_line<14, IMAG, METHOD>(ctx, b, n_steps); break;
+        default: launch_line<16, IMAG, METHOD>(ctx, b, n_steps); break;
     }
 }
 
+// complex time steps take the general instance (ADI only: the host rejects the
+// split operator with complex steps)
 void dispatch_line(tdse_ctx* ctx, const Bucket& b, int n_steps) {
-    if (b.imag) dispatch_line_m<false>(ctx, b, n_steps);
-    else dispatch_line_m<true>(ctx, b, n_steps);
+    if (b.imag) dispatch_line_m<false, LINE_ADI>(ctx, b, n_steps);
+    else if (b.method == LINE_SIO_LEN) dispatch_line_m<true, LINE_SIO_LEN>(ctx, b, n_steps);
+    else if (b.method == LINE_SIO_VEL) dispatch_line_m<true, LINE_SIO_VEL>(ctx, b, n_steps);
+    else dispatch_line_m<true, LINE_ADI>(ctx, b, n_steps);
 }
 
 template <int M, int MAXT, int WPE>
@@ -939,6 +945,9 @@ int tdse_line_add(tdse_ctx* ctx, const tdse_line_desc* desc, const double* table
     if (D.n_steps < 1) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: n_steps must be >= 1");
     if (D.store_every == 0 || D.store_every < -1) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: store_every must be >= 1 or -1");
     if (D.n_basis < 0 || (D.n_basis > 0 && !basis)) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: basis missing");
+    if (D.method < 0 || D.method > 2) return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: method must be 0, 1 or 2");
+    if (D.method != 0 && D.renormalize)
+        return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: the split operator does not renormalise");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     int s = -1;
     for (int i = 0; i < (int)ctx->line.size(); ++i)
@@ -956,6 +965,8 @@ int tdse_line_add(tdse_ctx* ctx, const tdse_line_desc* desc, const double* table
     H.complex_steps = false;
     for (int i = 0; i < D.n_steps; ++i)
         if (wt[2 * i] != 0.0) H.complex_steps = true;
+    if (D.method != 0 && H.complex_steps)
+        return fail(ctx, TDSE_ERR_ARG, "tdse_line_add: the split operator needs real time steps");
     LineSim& S = H.dev;
     S = LineSim{};
     S.n = D.z_points;
@@ -970,6 +981,8 @@ int tdse_line_add(tdse_ctx* ctx, const tdse_line_desc* desc, const double* table
     S.off = D.offdiag;
     S.zq0 = D.zq_first;
     S.dzq = D.zq_step;
+    S.vq = D.vel_factor;
+    S.method = D.method;
     S.step = 0;
     S.active = 1;
     H.n_data = n_data_points(S.N, S.store_every);

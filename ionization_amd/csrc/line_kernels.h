@@ -8,6 +8,14 @@
 // tau = (t_{n+1} - t_n) / 2 hbar (complex tau for imaginary-time relaxation)
 // and M the mask evaluated at the signed z (meshes.py:318-320).
 //
+// SplitInteractionOperator on a LineMesh (evolution_methods.py:89-123): the
+// split interaction factors, CN with the internal H0 = K + V, the factors in
+// reverse, then the mask. Length gauge (mesh_operators.py:329-342): the factor
+// is the diagonal phase exp(-i tau V_int), V_int = -z q E(t_{n+1}). Velocity
+// gauge (mesh_operators.py:352-427): real rotations by one angle
+// a = tau A(t) hbar (q / m) / (2 dz) on the z-pairs (k, k + 1) of even k,
+// then of odd k, A the vector potential -integral of E up to t_{n+1}.
+//
 // One workgroup owns one simulation's line for the whole launch: every thread
 // holds its chunk of m rows in registers and the kernel advances up to
 // n_steps steps before writing g back, so a whole run can be one launch.
@@ -27,12 +35,82 @@ struct LineSim {
     const double2* basis;  // [n_basis][npad] test states' g, position order
     double delta_z;
     double off, zq0, dzq;  // tau0 P_z; z coupling of row j = zq0 + j dzq (recomputed, not read)
+    double vq;             // velocity gauge: tau0 hbar (q / m) / (2 dz); the step's angle is (tau_n / tau0) A_n vq
     int n, P, m, npad, N, store_every, n_basis, renormalize;
+    int method;            // LINE_ADI, LINE_SIO_LEN or LINE_SIO_VEL (real time steps only for the split operator)
     int step;
     int active;
 };
 
 enum { LN_H = 0, LN_AP = 1, LN_AN = 2, LN_ZQ = 3, LN_MASK = 4, LN_NTAB = 5 };
+enum { LINE_ADI = 0, LINE_SIO_LEN = 1, LINE_SIO_VEL = 2 };
+
+// Length-gauge split factor: x_j *= exp(-i th_j), th_j = s (zq0 + j dzq) for the
+// thread's rows j = k M + i. th_j is linear in j, so the phase advances by one
+// complex multiplication per row from one sine and cosine at the chunk's first
+// row and one of the step th_{j+1} - th_j.
+template <int M>
+__device__ __forceinline__ void line_phase(double2 (&x)[M], const int k, const double s, const double zq0,
+                                           const double dzq) {
+    double s0, c0, sd, cd;
+    tdse_sincos(s * fma((double)(k * M), dzq, zq0), &s0, &c0);
+    tdse_sincos(s * dzq, &sd, &cd);
+    double2 ph = cmk(c0, -s0);
+    const double2 stp = cmk(cd, -sd);
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        x[i] = cmul(ph, x[i]);
+        ph = cmul(ph, stp);
+    }
+}
+
+// Velocity-gauge split factors: the rotation [[cos a, sin a], [-sin a, cos a]]
+// on the z-pairs (j, j + 1), j + 1 < nz, of even j then odd j (rev: odd, then
+// even). With M even the even pairs lie inside a thread; of the odd pairs the
+// one across a chunk edge uses the neighbouring thread's row through LDS.
+template <int M>
+__device__ __forceinline__ void line_vel_pairs(double2 (&x)[M], const int k, const int P,
+                                               double2* __restrict__ lds, const double a, const int nz,
+                                               const bool rev) {
+    double sn, cs;  // one angle for the whole line (uniform across the block)
+    if (fabs(a) <= kSmallAngle) tdse_sincos_small(a, &sn, &cs);
+    else tdse_sincos(a, &sn, &cs);
+    auto rot = [&](double2& u, double2& v) {
+        const double2 y0 = cmk(fma(cs, u.x, sn * v.x), fma(cs, u.y, sn * v.y));
+        const double2 y1 = cmk(fma(cs, v.x, -sn * u.x), fma(cs, v.y, -sn * u.y));
+        u = y0;
+        v = y1;
+    };
+    auto even = [&]() {
+#pragma unroll
+        for (int i = 0; i < M; i += 2)
+            if (k * M + i + 1 < nz) rot(x[i], x[i + 1]);
+    };
+    auto odd = [&]() {
+#pragma unroll
+        for (int i = 1; i + 1 < M; i += 2)
+            if (k * M + i + 1 < nz) rot(x[i], x[i + 1]);
+        __syncthreads();  // the LDS rows may still be read by the previous exchange
+        lds[k] = x[M - 1];
+        lds[P + k] = x[0];
+        __syncthreads();
+        if (k + 1 < P && (k + 1) * M < nz) {
+            double2 r = lds[P + k + 1];
+            rot(x[M - 1], r);
+        }
+        if (k > 0 && k * M < nz) {
+            double2 l = lds[k - 1];
+            rot(l, x[0]);
+        }
+    };
+    if (rev) {
+        odd();
+        even();
+    } else {
+        even();
+        odd();
+    }
+}
 
 __device__ __forceinline__ bool line_is_data_step(int n, int N, int every) {
     return n == 0 || n == N || (every >= 1 && (n % every) == 0);
@@ -82,7 +160,9 @@ __device__ __forceinline__ void line_observe(const LineSim& S, const double2 (&x
 // blockDim.x == P and m == M for every simulation launched together; one block per simulation.
 // IMAG: every w_n is purely imaginary (real time steps), which halves the
 // coefficient arithmetic; imaginary-time relaxation uses the general path.
-template <int M, int MAXT, bool IMAG>
+// METHOD: the step of every simulation of the launch (LINE_ADI, LINE_SIO_LEN,
+// LINE_SIO_VEL; the split operator only with IMAG).
+template <int M, int MAXT, bool IMAG, int METHOD>
 __global__ void __launch_bounds__(MAXT) k_line_steps(LineSim* __restrict__ sims, const int2* __restrict__ ids,
                                                      int n_ids, int n_steps) {
     extern __shared__ double2 lds[];  // 3 * P double2, then reduction scratch
@@ -108,9 +188,15 @@ __global__ void __launch_bounds__(MAXT) k_line_steps(LineSim* __restrict__ sims,
 
     if (n == 0 && line_is_data_step(0, N, S.store_every)) line_observe<M>(S, x, k, P, 0, red);
 
+    static_assert(IMAG || METHOD == LINE_ADI, "the split operator takes real time steps");
     for (int it = 0; it < n_steps && n < N; ++it, ++n) {
         const double2 w = ld2(S.wt, n);
-        const double f = ld1(S.fs, n);
+        const double fs = ld1(S.fs, n);
+        // split operator: the interaction factors before the CN half
+        if constexpr (METHOD == LINE_SIO_LEN) line_phase<M>(x, k, w.y * fs, S.zq0, S.dzq);
+        if constexpr (METHOD == LINE_SIO_VEL) line_vel_pairs<M>(x, k, P, lds, w.y * fs * S.vq, S.n, false);
+        // the interaction sits in the CN diagonal only for the ADI step
+        const double f = (METHOD == LINE_ADI) ? fs : 0.0;
         double2 xl, xr;
         chunk_neighbours<M>(x, k, P, lds, xl, xr, true);
         if constexpr (IMAG) {
@@ -151,6 +237,8 @@ __global__ void __launch_bounds__(MAXT) k_line_steps(LineSim* __restrict__ sims,
             const int ib = ((nz - 1) / M == k) ? (nz - 1) % M : -1;
             cn_implicit_lowreg<M, decltype(coef), decltype(coef2), true, 1, decltype(ratio)>(
                 x, k, P, coef, coef2, lds, xl, xr, 0, ratio, ib);
+            if constexpr (METHOD == LINE_SIO_LEN) line_phase<M>(x, k, sw * fs, S.zq0, S.dzq);
+            if constexpr (METHOD == LINE_SIO_VEL) line_vel_pairs<M>(x, k, P, lds, sw * fs * S.vq, nz, true);
         } else {
             auto coef = [&](int i, double& hm, double& h, double& hp) {
                 h = fma(f, ld1(ZQ, i * P), ld1(H, i * P));

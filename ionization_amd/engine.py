@@ -361,6 +361,8 @@ class CompiledLine:
     fs: np.ndarray  # (N,)
     g0: np.ndarray  # (npad,) complex
     basis: np.ndarray  # (nb, npad) complex
+    method: int = 0  # 0: ADI; 1: split operator, length gauge; 2: split operator, velocity gauge
+    vel_factor: float = 0.0  # method 2: tau0 hbar (q / m) / (2 dz)
 
     @property
     def npad(self):
@@ -379,14 +381,26 @@ class CompiledLine:
 
 def compile_line(*, z_bound, z_points, potential, field: Callable, mask, times, g0,
                  basis: Sequence[np.ndarray] = (), store_every=1, test_mass=u.electron_mass_reduced,
-                 test_charge=u.electron_charge, time_steps=None, renormalize=False, rows=None) -> CompiledLine:
+                 test_charge=u.electron_charge, time_steps=None, renormalize=False, rows=None,
+                 method: str = "adi", gauge: str = "length") -> CompiledLine:
     """
-    Compile a LineMesh + LineLengthGaugeOperators + ADI simulation
-    (meshes.py:263-325, mesh_operators.py:304-327). H = K + V(z) - z q E(t_{n+1});
-    tau0 = (t_1 - t_0) / 2 hbar. ``time_steps`` (complex) overrides the real
-    steps of ``times`` (imaginary-time relaxation); the mask is evaluated at
-    the signed z (meshes.py:318-320).
+    Compile a LineMesh simulation (meshes.py:263-325). ``method="adi"``:
+    LineLengthGaugeOperators + AlternatingDirectionImplicit, H = K + V(z) -
+    z q E(t_{n+1}) (mesh_operators.py:304-327). ``method="sio"``:
+    SplitInteractionOperator with the CN step in K + V and the interaction
+    split around it, ``gauge="length"`` (the phase exp(-i tau V_int),
+    mesh_operators.py:329-342) or ``gauge="velocity"`` (z-pair rotations by
+    tau A hbar (q / m) / (2 dz), A from ``vector_potential_table``,
+    mesh_operators.py:352-427). tau0 = (t_1 - t_0) / 2 hbar. ``time_steps``
+    (complex, ADI only) overrides the real steps of ``times`` (imaginary-time
+    relaxation); the mask is evaluated at the signed z (meshes.py:318-320).
     """
+    if method not in ("adi", "sio") or gauge not in ("length", "velocity"):
+        raise DeviceError(f"unknown LineMesh method {method!r} / gauge {gauge!r}")
+    if method == "adi" and gauge != "length":
+        raise DeviceError("the LineMesh ADI step implements the length gauge")
+    if method == "sio" and (time_steps is not None or renormalize):
+        raise DeviceError("the LineMesh split operator takes real time steps without renormalisation")
     P, m = choose_layout(z_points, "line", rows)
     z = np.linspace(-z_bound, z_bound, z_points)
     delta_z = np.abs(z[1] - z[0])
@@ -404,7 +418,14 @@ def compile_line(*, z_bound, z_points, potential, field: Callable, mask, times, 
     MK = np.asarray(mask(z), dtype=np.float64) * np.ones_like(z) if mask is not None else np.ones_like(z)
     tables = to_positions(np.stack([H, AP, AN, ZQ, MK]), P, m)
     wt = 1j * (steps / (2 * u.hbar)) / tau0
-    fs = np.asarray(field(times[1:]), dtype=np.float64) * np.ones(len(steps))
+    code, vel_factor = 0, 0.0
+    if method == "sio" and gauge == "velocity":
+        code = 2
+        vel_factor = float(tau0 * u.hbar * (test_charge / test_mass) / (2 * delta_z))
+        fs = vector_potential_table(field, times)
+    else:
+        code = 1 if method == "sio" else 0
+        fs = np.asarray(field(times[1:]), dtype=np.float64) * np.ones(len(steps))
     bmat = np.array([np.asarray(b, dtype=np.complex128) for b in basis]).reshape(len(basis), z_points)
     return CompiledLine(
         z_points=z_points, P=P, m=m, n_steps=len(steps), store_every=int(store_every), delta_z=delta_z,
@@ -412,7 +433,7 @@ def compile_line(*, z_bound, z_points, potential, field: Callable, mask, times, 
         zq_step=float(tau0 * (-test_charge) * (z[1] - z[0])),
         tables=np.ascontiguousarray(tables), wt=np.ascontiguousarray(wt),
         fs=np.ascontiguousarray(fs), g0=np.ascontiguousarray(to_positions(np.asarray(g0, np.complex128), P, m)),
-        basis=np.ascontiguousarray(to_positions(bmat, P, m)),
+        basis=np.ascontiguousarray(to_positions(bmat, P, m)), method=code, vel_factor=vel_factor,
     )
 
 
@@ -661,8 +682,8 @@ class DeviceContext:
         return slot.value
 
     def add_line(self, c: CompiledLine) -> int:
-        desc = _lib.LineDesc(c.z_points, c.P, c.m, c.n_steps, c.store_every, len(c.basis), int(c.renormalize), 0,
-                             c.delta_z, c.offdiag, c.zq_first, c.zq_step)
+        desc = _lib.LineDesc(c.z_points, c.P, c.m, c.n_steps, c.store_every, len(c.basis), int(c.renormalize),
+                             int(c.method), c.delta_z, c.offdiag, c.zq_first, c.zq_step, c.vel_factor)
         slot = ctypes.c_int32()
         basis = np.ascontiguousarray(c.basis, dtype=np.complex128) if len(c.basis) else None
         self._check(self._lib.tdse_line_add(

@@ -5,6 +5,7 @@ from .mesh_operators import (
     MeshOperators,
     KineticEnergyDerivation,
     LineLengthGaugeOperators,
+    LineVelocityGaugeOperators,
     CylindricalSliceLengthGaugeOperators,
     SphericalHarmonicLengthGaugeOperators,
     SphericalHarmonicVelocityGaugeOperators,

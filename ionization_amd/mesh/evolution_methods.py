@@ -13,7 +13,7 @@ import numpy as np
 
 from .. import core, engine, potentials
 from .. import units as u
-from . import data, meshes
+from . import data, meshes, mesh_operators
 
 
 class EvolutionMethod:
@@ -62,8 +62,12 @@ class SplitInteractionOperator(EvolutionMethod):
 
     def compile(self, sim, g0=None):
         mesh = sim.mesh
+        if isinstance(mesh, meshes.LineMesh):
+            # LineMesh: the interaction split around a CN step in K + V (mesh_operators.py:329-342, 352-427)
+            velocity = isinstance(sim.spec.operators, mesh_operators.LineVelocityGaugeOperators)
+            return _compile_line(sim, g0, method="sio", gauge="velocity" if velocity else "length")
         if not isinstance(mesh, meshes.SphericalHarmonicMesh):
-            raise NotImplementedError("SplitInteractionOperator runs on SphericalHarmonicMesh only")
+            raise NotImplementedError("SplitInteractionOperator runs on SphericalHarmonicMesh and LineMesh")
         gauge = sim.spec.operators.gauge
         if gauge not in (core.Gauge.LENGTH, core.Gauge.VELOCITY):
             raise NotImplementedError(f"unknown gauge {gauge}")
@@ -98,17 +102,11 @@ class AlternatingDirectionImplicit(EvolutionMethod):
     def compile(self, sim, g0=None):
         mesh = sim.mesh
         if isinstance(mesh, meshes.LineMesh):
-            if sim.spec.operators.gauge != core.Gauge.LENGTH:
-                raise NotImplementedError("the device LineMesh step implements the length gauge")
-            basis = [np.asarray(mesh.get_g_for_state(s), dtype=np.complex128) for s in sim.spec.test_states]
-            return engine.compile_line(
-                z_bound=sim.spec.z_bound, z_points=sim.spec.z_points,
-                potential=lambda z: np.asarray(sim.spec.internal_potential(r=z, z=z, test_charge=sim.spec.test_charge),
-                                               dtype=np.float64),
-                field=_field(sim), mask=_mask(sim), times=sim.times, g0=mesh.g if g0 is None else g0, basis=basis,
-                store_every=sim.spec.store_data_every, test_mass=sim.spec.test_mass,
-                test_charge=sim.spec.test_charge, rows=getattr(sim.spec, "device_rows", None),
-            )
+            if sim.spec.operators.gauge != core.Gauge.LENGTH or isinstance(
+                    sim.spec.operators, mesh_operators.LineVelocityGaugeOperators):
+                raise NotImplementedError("the device LineMesh ADI step implements the length gauge "
+                                          "(the velocity gauge runs with SplitInteractionOperator)")
+            return _compile_line(sim, g0, method="adi", gauge="length")
         if isinstance(mesh, meshes.CylindricalSliceMesh):
             if sim.spec.operators.gauge != core.Gauge.LENGTH:
                 raise NotImplementedError("the device CylindricalSliceMesh step implements the length gauge")
@@ -121,6 +119,19 @@ class AlternatingDirectionImplicit(EvolutionMethod):
                 test_charge=sim.spec.test_charge, mask=_radial_mask(sim.spec.mask),
             )
         raise NotImplementedError(f"device ADI is not implemented for {mesh.__class__.__name__}")
+
+
+def _compile_line(sim, g0, *, method, gauge):
+    mesh = sim.mesh
+    basis = [np.asarray(mesh.get_g_for_state(s), dtype=np.complex128) for s in sim.spec.test_states]
+    return engine.compile_line(
+        z_bound=sim.spec.z_bound, z_points=sim.spec.z_points,
+        potential=lambda z: np.asarray(sim.spec.internal_potential(r=z, z=z, test_charge=sim.spec.test_charge),
+                                       dtype=np.float64),
+        field=_field(sim), mask=_mask(sim), times=sim.times, g0=mesh.g if g0 is None else g0, basis=basis,
+        store_every=sim.spec.store_data_every, test_mass=sim.spec.test_mass,
+        test_charge=sim.spec.test_charge, rows=getattr(sim.spec, "device_rows", None), method=method, gauge=gauge,
+    )
 
 
 def _coulomb_charge(pot):

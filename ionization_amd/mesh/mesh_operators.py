@@ -130,6 +130,19 @@ class LineLengthGaugeOperators(MeshOperators):
         return (-2 * prefactor) * np.ones(n) + v * np.ones(n), prefactor * np.ones(n - 1)
 
 
+class LineVelocityGaugeOperators(LineLengthGaugeOperators):
+    """
+    Velocity-gauge interaction of a LineMesh (mesh_operators.py:352-427), for
+    SplitInteractionOperator: A(t) i hbar (q / m) / (2 dz) above the
+    diagonal and its negative below, split into rotations of the even and odd
+    z-pairs. The reference labels this class with the length gauge
+    (mesh_operators.py:355); the attribute is kept as it is there and the
+    device path recognises the class itself.
+    """
+
+    gauge = core.Gauge.LENGTH
+
+
 class CylindricalSliceLengthGaugeOperators(MeshOperators):
     """Length-gauge operators of a CylindricalSliceMesh (mesh_operators.py:550-631)."""
 

@@ -512,6 +512,66 @@ class LineOracle:
         return g, np.array(norms), np.array(ips, dtype=np.complex128)
 
 
+class LineSIOOracle(LineOracle):
+    """
+    LineMesh evolved by SplitInteractionOperator (evolution_methods.py:89-123): the split
+    interaction factors, CN explicit and implicit with the internal Hamiltonian K + V
+    (mesh_operators.py:244-269), the factors reversed, then the mask.
+    Length gauge (LineLengthGaugeOperators.split_interaction_operators,
+    mesh_operators.py:329-342): one diagonal factor exp(-i tau V_int), V_int = -z q E(t_{n+1}).
+    Velocity gauge (LineVelocityGaugeOperators, mesh_operators.py:352-427): h = A(t) i hbar
+    (q / m) / (2 dz) (super - sub diagonal), A = -simps(E, times_to_current); factors: the
+    real rotations on the z-pairs (k, k + 1) of even k, then of odd k.
+    """
+
+    def __init__(self, *, gauge="length", **kw):
+        super().__init__(**kw)
+        self.gauge = gauge
+
+    def split_operators(self, tau, time_now, times_to_current):
+        n = len(self.z)
+        if self.gauge == "length":
+            epot = -self.z * self.test_charge * self.field(time_now)
+            return [_Op("dot", sparse.diags((np.exp(-1j * epot * tau),), offsets=(0,)), None)]
+        apot = -simps_old(self.field(times_to_current), times_to_current)
+        off = (1j * u.hbar * (self.test_charge / self.test_mass) / (2 * self.delta_z)) * np.ones(n - 1)
+        h = sparse.diags((-(apot * off), apot * off), offsets=(-1, 1))
+        a = h.data[-1][1:] * tau * (-1j)
+        ops = []
+        for parity in (0, 1):
+            diag = np.ones(n, dtype=np.complex128)
+            offd = np.zeros(n - 1, dtype=np.complex128)
+            ks = np.arange(parity, n - 1, 2)
+            diag[ks] = np.cos(a[ks])
+            diag[ks + 1] = np.cos(a[ks])
+            offd[ks] = np.sin(a[ks])
+            ops.append(_Op("dot", sparse.diags((-offd, diag, offd), offsets=(-1, 0, 1)), None))
+        return ops
+
+    def evolve_sio(self, g, time_step, time_now, times_to_current):
+        tau = time_step / (2 * u.hbar)
+        cn = [_Op("dot", add_to_diagonal((-1j * tau) * self.h_internal, value=1), None),
+              _Op("tdma", add_to_diagonal((1j * tau) * self.h_internal, value=1), None)]
+        split = self.split_operators(tau, time_now, times_to_current)
+        g = apply_sequentially(lambda m, d: m, lambda v, d: v, g, (*split, *cn, *reversed(split)))
+        if self.mask_mesh is not None:
+            g = g * self.mask_mesh
+        return g
+
+    def run(self, g0, times, store_every=1, basis=()):
+        g = np.array(g0, dtype=np.complex128)
+        didx = set(data_indices(len(times), store_every).tolist())
+        norms, ips = [], []
+        for i in range(len(times)):
+            if i in didx:
+                norms.append(self.norm(g))
+                ips.append([self.inner_product(phi, g) for phi in basis])
+            if i == len(times) - 1:
+                break
+            g = self.evolve_sio(g, times[i + 1] - times[i], times[i + 1], times[: i + 2])
+        return g, np.array(norms), np.array(ips, dtype=np.complex128)
+
+
 # ---------------------------------------------------------------------------
 # CylindricalSliceMesh + CylindricalSliceLengthGaugeOperators + ADI
 

@@ -50,19 +50,24 @@ def sh_case(*, r_bound_bohr=30.0, r_points=200, l_bound=20, pulse_width_as=100.0
 
 
 def line_case(*, z_bound_bohr=50.0, z_points=500, n_steps=300, e0_au=0.05, omega_au=0.057, dt_au=0.0413,
-              store_every=1, mask=True, rows=None):
-    """Soft-Coulomb LineMesh CN with a sine field and a one-sided absorber (mask at signed z)."""
+              store_every=1, mask=True, rows=None, method="adi", gauge="length", ramp=True):
+    """Soft-Coulomb LineMesh (ADI, or the split operator in either gauge) with a sine field and a
+    one-sided absorber (mask at signed z)."""
     z_bound = z_bound_bohr * u.bohr_radius
     soft = np.sqrt(2.0) * u.bohr_radius
     pot = lambda z: physics.soft_coulomb(z, soft)
     omega = omega_au / u.atomic_time
     period = u.twopi / omega
-    field = lambda t: physics.sine_wave_field(t, omega=omega, amplitude=e0_au * u.atomic_electric_field,
-                                              time_front=2 * period, time_plateau=10 * period)
+    window = dict(time_front=2 * period, time_plateau=10 * period) if ramp else {}
+    field = lambda t: physics.sine_wave_field(t, omega=omega, amplitude=e0_au * u.atomic_electric_field, **window)
     maskf = (lambda z: physics.radial_cosine_mask(z, 0.8 * z_bound, z_bound)) if mask else None
     mass = u.electron_mass
-    orc = mesh_cpu.LineOracle(z_bound=z_bound, z_points=z_points, potential=pot, field=field, mask=maskf,
-                              test_mass=mass)
+    if method == "sio":
+        orc = mesh_cpu.LineSIOOracle(z_bound=z_bound, z_points=z_points, potential=pot, field=field, mask=maskf,
+                                     test_mass=mass, gauge=gauge)
+    else:
+        orc = mesh_cpu.LineOracle(z_bound=z_bound, z_points=z_points, potential=pot, field=field, mask=maskf,
+                                  test_mass=mass)
     hd = orc.h_internal.diagonal().real
     ho = orc.h_internal.diagonal(1).real
     w, v = sl.eigh_tridiagonal(hd, ho, select="i", select_range=(0, 1))
@@ -71,7 +76,8 @@ def line_case(*, z_bound_bohr=50.0, z_points=500, n_steps=300, e0_au=0.05, omega
     times = np.linspace(0.0, n_steps * dt, n_steps + 1)
     basis = [g0.astype(np.complex128), (v[:, 1] / np.sqrt(orc.delta_z * np.sum(v[:, 1] ** 2))).astype(np.complex128)]
     comp = engine.compile_line(z_bound=z_bound, z_points=z_points, potential=pot, field=field, mask=maskf,
-                               times=times, g0=g0, basis=basis, store_every=store_every, test_mass=mass, rows=rows)
+                               times=times, g0=g0, basis=basis, store_every=store_every, test_mass=mass, rows=rows,
+                               method=method, gauge=gauge)
     return dict(oracle=orc, compiled=comp, g0=g0.astype(np.complex128), times=times, basis=basis,
                 store_every=store_every)
 

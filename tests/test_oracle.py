@@ -495,3 +495,77 @@ def test_simps_old_exact_for_cubics_on_uniform_odd_grids():
     y = 2 * x ** 3 - x ** 2 + 3
     exact = (2 * 2 ** 4 / 4 - 2 ** 3 / 3 + 6) - (2 / 4 + 1 / 3 - 3)
     assert np.isclose(mesh_cpu.simps_old(y, x), exact, rtol=1e-14)
+
+
+# ------------------------------------------------------------------ LineMesh split operator
+
+
+def _line_sio(n, gauge, e0=0.3):
+    return mesh_cpu.LineSIOOracle(
+        z_bound=20 * u.bohr_radius, z_points=n, potential=lambda z: physics.soft_coulomb(z, np.sqrt(2) * u.bohr_radius),
+        field=lambda t: e0 * u.atomic_electric_field * np.sin(t / (30 * u.asec)), gauge=gauge,
+        test_mass=u.electron_mass)
+
+
+@pytest.mark.parametrize("n", [9, 10])
+def test_line_velocity_split_factors_are_exponentials(n):
+    """LineVelocityGaugeOperators (mesh_operators.py:352-427): the even / odd factors are
+    exp(-i tau h_part) of h = A i hbar (q / m) / (2 dz) (super - sub diagonal) restricted to the
+    pairs (k, k + 1) of even / odd k. A strong field makes the angles O(1)."""
+    o = _line_sio(n, "velocity", e0=40.0)
+    ts = np.arange(8) * 5 * u.asec
+    tau = 5 * u.asec / (2 * u.hbar)
+    A = -mesh_cpu.simps_old(o.field(ts), ts)
+    pref = A * 1j * u.hbar * (u.electron_charge / u.electron_mass) / (2 * o.delta_z)
+    H = np.diag(np.full(n - 1, pref), 1) - np.diag(np.full(n - 1, pref), -1)
+    ops = o.split_operators(tau, ts[-1], ts)
+    assert 0.1 < abs(tau * pref) < 3.0
+    rng = np.random.default_rng(n)
+    g = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    for op, par in zip(ops, (0, 1)):
+        part = np.zeros_like(H)
+        for k in range(par, n - 1, 2):
+            part[k, k + 1], part[k + 1, k] = H[k, k + 1], H[k + 1, k]
+        assert rel_l2(op.matrix.dot(g), sl.expm(-1j * tau * part) @ g) < 1e-13
+
+
+def test_line_length_split_step_matches_dense_composition():
+    """Line SIO, length gauge: D . CN(K + V) . D . mask with D = exp(-i tau V_int(t_{n+1}))
+    (mesh_operators.py:329-342, evolution_methods.py:113-121)."""
+    n = 12
+    maskf = lambda z: physics.radial_cosine_mask(z, 10 * u.bohr_radius, 20 * u.bohr_radius)
+    o = mesh_cpu.LineSIOOracle(z_bound=20 * u.bohr_radius, z_points=n,
+                               potential=lambda z: physics.soft_coulomb(z, np.sqrt(2) * u.bohr_radius),
+                               field=lambda t: 2.0 * u.atomic_electric_field * np.cos(t / (40 * u.asec)),
+                               mask=maskf, test_mass=u.electron_mass, gauge="length")
+    rng = np.random.default_rng(4)
+    g = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    dt, t1 = 3 * u.asec, 9 * u.asec
+    out = o.evolve_sio(g, dt, t1, np.array([t1 - dt, t1]))
+    tau = dt / (2 * u.hbar)
+    H0 = o.h_internal.toarray()
+    D = np.diag(np.exp(-1j * tau * (-o.z * u.electron_charge * o.field(t1))))
+    cn = np.linalg.solve(np.eye(n) + 1j * tau * H0, np.eye(n) - 1j * tau * H0)
+    ref = (D @ cn @ D @ g) * maskf(o.z)
+    assert rel_l2(out, ref) < 1e-12
+
+
+def test_line_split_operator_without_field_is_the_adi_step():
+    """E = 0: the split factors are the identity and both gauges take the ADI step in K + V."""
+    kw = dict(z_bound=20 * u.bohr_radius, z_points=11, field=lambda t: 0.0 * t, test_mass=u.electron_mass,
+              potential=lambda z: physics.soft_coulomb(z, np.sqrt(2) * u.bohr_radius))
+    rng = np.random.default_rng(6)
+    g = rng.standard_normal(11) + 1j * rng.standard_normal(11)
+    times = np.arange(4) * 2 * u.asec
+    adi = mesh_cpu.LineOracle(**kw).run(g, times)[0]
+    for gauge in ("length", "velocity"):
+        assert rel_l2(mesh_cpu.LineSIOOracle(gauge=gauge, **kw).run(g, times)[0], adi) < 1e-14
+
+
+@pytest.mark.parametrize("gauge", ["length", "velocity"])
+def test_line_split_operator_is_unitary_without_mask(gauge):
+    o = _line_sio(40, gauge)
+    rng = np.random.default_rng(7)
+    g = rng.standard_normal(40) + 1j * rng.standard_normal(40)
+    gf, norms, _ = o.run(g, np.arange(30) * 2 * u.asec)
+    assert abs(norms[-1] - norms[0]) < 1e-12 * norms[0]
