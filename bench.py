@@ -192,12 +192,26 @@ def time_line(n_batch, cpu_seconds):
     ms_batch = ctx.line_run(comp.n_steps)
     for s in slots:
         ctx.remove(KIND_LINE, s)
+    # the same workload with SplitInteractionOperator (the reference benchmark's Line_SO_LEN / Line_SO_VEL)
+    from ionization_amd import mesh
+
+    so = {}
+    for gauge, ops in (("length", mesh.LineLengthGaugeOperators()), ("velocity", mesh.LineVelocityGaugeOperators())):
+        sspec = workloads.line_cn_spec(z_points=8192, n_steps=4000, store_data_every=100,
+                                       ground_state=np.asarray(spec.initial_state.g), operators=ops,
+                                       evolution_method=mesh.SplitInteractionOperator())
+        ssim = sspec.to_sim()
+        sc = sspec.evolution_method.compile(ssim)
+        s1 = ctx.add_line(sc)
+        ms = ctx.line_run(sc.n_steps)
+        ctx.remove(KIND_LINE, s1)
+        so[gauge] = {"single_ms": ms, "single_value": sc.z_points * sc.n_steps / (ms * 1e-3)}
     ctx.close()
     work = comp.z_points * comp.n_steps
     out = {"workload": "line_cn: LineMesh 8192 points x 4000 steps, soft Coulomb, sine field, CN (one launch)",
            "single_ms": ms_single, "single_value": work / (ms_single * 1e-3),
            "batch": n_batch, "batch_ms": ms_batch, "batch_value": n_batch * work / (ms_batch * 1e-3),
-           "unit": "mesh-point-steps/s"}
+           "unit": "mesh-point-steps/s", "split_operator": so}
     if cpu_seconds > 0:
         from oracle import mesh_cpu
 
@@ -213,7 +227,8 @@ def time_line(n_batch, cpu_seconds):
             steps += 1
         out["cpu_port_value"] = comp.z_points * steps / (time.perf_counter() - t0)
     log(f"line: single {ms_single:.1f} ms ({out['single_value'] / 1e9:.3f} G/s), "
-        f"batch of {n_batch} {ms_batch:.1f} ms ({out['batch_value'] / 1e9:.2f} G/s)")
+        f"batch of {n_batch} {ms_batch:.1f} ms ({out['batch_value'] / 1e9:.2f} G/s); split operator single "
+        f"{so['length']['single_ms']:.1f} ms (length), {so['velocity']['single_ms']:.1f} ms (velocity)")
     return out
 
 
@@ -429,6 +444,80 @@ def summarize_profiles(d, command_trace, command_fetch, command_write, trace_ste
     print(open(TRAFFIC_FILE).read())
 
 
+def summarize_velocity_profiles(d, command_trace, command_fetch, command_write, sims=256, seed=0,
+                                trace_steps=20, trace_warmup=5, counter_steps=3, counter_warmup=1):
+    """Velocity-gauge scan: kernel trace by full kernel name (the velocity L-phase is the
+    instance k_sh_lphase<*, true>) from d/vtrace, and the h2 kernel's calibrated HBM traffic per
+    step from d/vfetch and d/vwrite against its algorithmic bytes."""
+    import glob
+    import sqlite3
+
+    from ionization_amd import workloads
+
+    def db(sub):
+        files = sorted(glob.glob(os.path.join(d, sub, "**", "*_results.db"), recursive=True))
+        if not files:
+            raise SystemExit(f"no rocprofv3 database under {os.path.join(d, sub)}")
+        return sqlite3.connect(files[0])
+
+    prof_dir = os.path.join(ROOT, "profiles")
+    con = db("vtrace")
+    rows = con.execute("select name, count(*), sum(duration), avg(duration), min(duration), max(duration) "
+                       "from kernels group by name order by sum(duration) desc").fetchall()
+    steps = trace_warmup + 2 * trace_steps  # warm-up, graph replays, profiled direct launches
+    lines = ["# rocprofv3 kernel trace of bench.py with the velocity-gauge scan (the same 256 draws,",
+             "# SphericalHarmonicVelocityGaugeOperators); the length-gauge scan runs first in the same process",
+             "", "command:", "    " + command_trace, "",
+             f"{'kernel':<72} {'calls':>7} {'total ms':>10} {'avg us':>9} {'min us':>9} {'max us':>9}"]
+    vph = [0, 0.0]
+    for name, cnt, tot, avg, mn, mx in rows:
+        if _kernel_family(name) == "k_sh_vphase":
+            vph[0] += cnt
+            vph[1] += tot
+        if tot < 1e5:
+            continue
+        lines.append(f"{name[:72]:<72} {cnt:>7d} {tot / 1e6:>10.3f} {avg / 1e3:>9.1f} {mn / 1e3:>9.1f} {mx / 1e3:>9.1f}")
+    lines += ["", f"k_sh_vphase, all instances: {vph[0]} launches, {vph[1] / 1e6:.3f} ms over {steps} steps = "
+              f"{vph[1] / 1e6 / steps:.3f} ms per step"]
+    with open(os.path.join(prof_dir, "sh_velocity_scan_kernel_trace.txt"), "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+    def counter(sub, cname):
+        q = ("select kernel_name, count(*), sum(value) from counters_collection "
+             "where counter_name = ? group by kernel_name")
+        return {k: (c, v * 1024.0) for k, c, v in db(sub).execute(q, (cname,)).fetchall()}
+
+    fetch, write = counter("vfetch", "FETCH_SIZE"), counter("vwrite", "WRITE_SIZE")
+
+    def cal(table):
+        for k, (c, v) in table.items():
+            if _kernel_family(k) == "k_stream_copy":
+                return COPY_CAL_BYTES * c / v
+        raise SystemExit("calibration copy kernel not found (run the profiled command with --calibrate-copy)")
+
+    fs, ws = cal(fetch), cal(write)
+    csteps = counter_warmup + 2 * counter_steps
+    alg = sum(2 * 32 * p["r_points"] * (2 * (p["l_bound"] // 2) + 2 * ((p["l_bound"] - 1) // 2))
+              for p in workloads.sh_scan_params(sims, seed))  # h2 rows read + written per step
+    fb = sum(v for k, (c, v) in fetch.items() if _kernel_family(k) == "k_sh_vphase") * fs / csteps
+    wb = sum(v for k, (c, v) in write.items() if _kernel_family(k) == "k_sh_vphase") * ws / csteps
+    lv = [(k, c, v) for k, (c, v) in fetch.items() if k.startswith("void k_sh_lphase<false, true>")]
+    lw = [(k, c, v) for k, (c, v) in write.items() if k.startswith("void k_sh_lphase<false, true>")]
+    lines = ["# HBM traffic of the velocity-gauge scan from rocprofv3 FETCH_SIZE / WRITE_SIZE (KiB per",
+             "# dispatch), scaled by the same counters on the library's 1 GiB copy kernel", "",
+             "commands:", "    " + command_fetch, "    " + command_write, "",
+             f"FETCH_SIZE scale = {fs:.4f}, WRITE_SIZE scale = {ws:.4f}; {csteps} steps counted", "",
+             f"k_sh_vphase (h2 factors, all instances) per step: fetch {fb / 1e6:.1f} MB, write {wb / 1e6:.1f} MB; "
+             f"algorithmic {alg / 2e6:.1f} MB each way ({fb / (alg / 2):.2f}x / {wb / (alg / 2):.2f}x)"]
+    if lv and lw:
+        lines.append(f"k_sh_lphase<false, true> (velocity L-phase) per launch: fetch "
+                     f"{lv[0][2] / lv[0][1] * fs / 1e6:.1f} MB, write {lw[0][2] / lw[0][1] * ws / 1e6:.1f} MB")
+    with open(os.path.join(prof_dir, "sh_velocity_scan_hbm_traffic.txt"), "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    print(open(os.path.join(prof_dir, "sh_velocity_scan_kernel_trace.txt")).read())
+    print(open(os.path.join(prof_dir, "sh_velocity_scan_hbm_traffic.txt")).read())
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -453,10 +542,15 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-pool-gb", type=float, default=96.0)
     ap.add_argument("--summarize-profiles", metavar="DIR")
+    ap.add_argument("--summarize-velocity-profiles", metavar="DIR")
     ap.add_argument("--trace-command", default="")
     ap.add_argument("--fetch-command", default="")
     ap.add_argument("--write-command", default="")
     args = ap.parse_args()
+    if args.summarize_velocity_profiles:
+        summarize_velocity_profiles(args.summarize_velocity_profiles, args.trace_command, args.fetch_command,
+                                    args.write_command, args.sims, args.seed, args.steps, args.warmup)
+        return
     if args.summarize_profiles:
         summarize_profiles(args.summarize_profiles, args.trace_command, args.fetch_command, args.write_command,
                            args.steps, args.warmup, args.cyl_steps)

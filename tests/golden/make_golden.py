@@ -12,6 +12,9 @@ Cases (parameters are stored in the file; the builders are tests/cases.py):
   sh    SphericalHarmonicMesh split operator, 120 r x 12 l, 80 steps, data every 5
   line  LineMesh CN, 300 z, 80 steps, data every 10
   cyl   CylindricalSliceMesh ADI, 64 z x 33 rho, 20 steps, data every 4
+  shv   SphericalHarmonicMesh velocity gauge, 61 r x 9 l, 60 steps, data every 5
+  lsl   LineMesh split operator, length gauge, 301 z, 60 steps, strong unramped field
+  lsv   LineMesh split operator, velocity gauge, 301 z, 60 steps, strong unramped field
 """
 
 import json
@@ -31,6 +34,9 @@ PARAMS = {
     "sh": dict(r_points=120, l_bound=12, t_half_as=40.0, store_every=5),
     "line": dict(z_points=300, n_steps=80, store_every=10),
     "cyl": dict(z_points=64, rho_points=33, t_half_as=10.0, store_every=4),
+    "shv": dict(r_points=61, l_bound=9, t_half_as=30.0, store_every=5),
+    "lsl": dict(z_points=301, n_steps=60, store_every=10, e0_au=2.0, omega_au=0.5),
+    "lsv": dict(z_points=301, n_steps=60, store_every=10, e0_au=2.0, omega_au=0.5),
 }
 
 
@@ -65,6 +71,20 @@ def build():
                     store_every=p["store_every"])
     g, norms, ips = case["oracle"].run(case["g0"], case["times"], p["store_every"], basis=case["basis"])
     out.update(cyl_g0=case["g0"], cyl_g=g, cyl_norms=norms, cyl_ips=ips)
+
+    p = PARAMS["shv"]
+    case = sh_case(r_points=p["r_points"], l_bound=p["l_bound"], t_half_as=p["t_half_as"], store_every=p["store_every"],
+                   gauge="velocity")
+    g, norms, ips, lnorm = case["oracle"].run(case["g0"], case["times"], case["dt"], p["store_every"],
+                                             basis=case["basis"])
+    out.update(shv_g0=case["g0"], shv_g=g, shv_norms=norms, shv_ips=ips, shv_lnorms=lnorm)
+
+    for key, gauge in (("lsl", "length"), ("lsv", "velocity")):
+        p = PARAMS[key]
+        case = line_case(z_points=p["z_points"], n_steps=p["n_steps"], store_every=p["store_every"], method="sio",
+                         gauge=gauge, e0_au=p["e0_au"], omega_au=p["omega_au"], ramp=False)
+        g, norms, ips = case["oracle"].run(case["g0"], case["times"], p["store_every"], basis=case["basis"])
+        out.update({f"{key}_g0": case["g0"], f"{key}_g": g, f"{key}_norms": norms, f"{key}_ips": ips})
     return out
 
 

@@ -67,3 +67,26 @@ def test_cyl_fixture(gold):
     g, norms, ips = case["oracle"].run(case["g0"], case["times"], p["store_every"], basis=case["basis"])
     assert rel_l2(g, gold["cyl_g"]) <= 1e-13
     assert np.max(np.abs(ips - gold["cyl_ips"])) <= 1e-13
+
+
+def test_sh_velocity_fixture(gold):
+    p = gold["params"]["shv"]
+    case = sh_case(r_points=p["r_points"], l_bound=p["l_bound"], t_half_as=p["t_half_as"], store_every=p["store_every"],
+                   gauge="velocity")
+    assert np.array_equal(case["g0"], gold["shv_g0"])
+    g, norms, ips, lnorm = case["oracle"].run(case["g0"], case["times"], case["dt"], p["store_every"],
+                                             basis=case["basis"])
+    assert rel_l2(g, gold["shv_g"]) <= 1e-13
+    assert np.max(np.abs(norms - gold["shv_norms"])) <= 1e-13
+    assert np.max(np.abs(ips - gold["shv_ips"])) <= 1e-13
+
+
+@pytest.mark.parametrize("key, gauge", [("lsl", "length"), ("lsv", "velocity")])
+def test_line_split_operator_fixture(gold, key, gauge):
+    p = gold["params"][key]
+    case = line_case(z_points=p["z_points"], n_steps=p["n_steps"], store_every=p["store_every"], method="sio",
+                     gauge=gauge, e0_au=p["e0_au"], omega_au=p["omega_au"], ramp=False)
+    assert rel_l2(case["g0"], gold[f"{key}_g0"]) <= 1e-15
+    g, norms, ips = case["oracle"].run(case["g0"], case["times"], p["store_every"], basis=case["basis"])
+    assert rel_l2(g, gold[f"{key}_g"]) <= 1e-13
+    assert np.max(np.abs(norms - gold[f"{key}_norms"])) <= 1e-13

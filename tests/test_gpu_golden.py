@@ -71,3 +71,31 @@ def test_cyl_matches_golden(device_ctx, gold):
     assert rel_l2(g, gold["cyl_g"]) <= WF_RTOL
     assert np.max(np.abs(norms - gold["cyl_norms"])) <= OBS_ATOL
     assert np.max(np.abs(ips - gold["cyl_ips"])) <= OBS_ATOL
+
+
+def test_sh_velocity_matches_golden(device_ctx, gold):
+    p = gold["params"]["shv"]
+    c = sh_case(r_points=p["r_points"], l_bound=p["l_bound"], t_half_as=p["t_half_as"],
+                store_every=p["store_every"], gauge="velocity")["compiled"]
+    s = device_ctx.add_sh(c)
+    device_ctx.run(c.n_steps + 1)
+    r = device_ctx.download_sh(s)
+    device_ctx.remove(0, s)
+    assert rel_l2(r.g, gold["shv_g"]) <= WF_RTOL
+    assert np.max(np.abs(r.norms - gold["shv_norms"])) <= OBS_ATOL
+    assert np.max(np.abs(r.lnorms - gold["shv_lnorms"])) <= OBS_ATOL
+    assert np.max(np.abs(r.ips - gold["shv_ips"])) <= OBS_ATOL
+
+
+@pytest.mark.parametrize("key, gauge", [("lsl", "length"), ("lsv", "velocity")])
+def test_line_split_operator_matches_golden(device_ctx, gold, key, gauge):
+    p = gold["params"][key]
+    c = line_case(z_points=p["z_points"], n_steps=p["n_steps"], store_every=p["store_every"], method="sio",
+                  gauge=gauge, e0_au=p["e0_au"], omega_au=p["omega_au"], ramp=False)["compiled"]
+    s = device_ctx.add_line(c)
+    device_ctx.line_run(c.n_steps)
+    g, norms, ips = device_ctx.download_line(s)
+    device_ctx.remove(KIND_LINE, s)
+    assert rel_l2(g, gold[f"{key}_g"]) <= WF_RTOL
+    assert np.max(np.abs(norms - gold[f"{key}_norms"])) <= OBS_ATOL
+    assert np.max(np.abs(ips - gold[f"{key}_ips"])) <= OBS_ATOL
