@@ -72,6 +72,7 @@ def time_batched(compiled, steps, warmup):
     ctx.run(steps, use_graph=False)
     prof = ctx.profile_read()
     prof["vphase_ms"], prof["vphase_launches"] = ctx.profile_kernel(4)
+    prof["rpair_ms"], prof["rpair_launches"] = ctx.profile_kernel(5)
     ctx.profile(False)
     used, total = ctx.pool_usage()
     ctx.close()
@@ -82,8 +83,10 @@ def time_velocity(params, store_every, steps, warmup):
     """
     The same scan draws in the velocity gauge (SphericalHarmonicVelocityGaugeOperators +
     SplitInteractionOperator): per step the L-phase (h1 rotations, mask), the h2
-    similarity factors of the even and the odd l-pairs, the R-phase, and the h2 factors
-    again in reverse: six read+write sweeps of g (192 B per mesh point and step).
+    similarity factors of the even l-pairs, the R-phase (the odd l-pairs solved together
+    with their h2 factors where two lines fit one workgroup), and the even pairs' factors
+    again: four read+write sweeps of g (128 B per mesh point and step; six where the odd
+    pairs' factors run as kernels of their own).
     """
     from ionization_amd import mesh, workloads
 
@@ -97,15 +100,20 @@ def time_velocity(params, store_every, steps, warmup):
     b = time_batched(compiled, steps, warmup)
     rate = b["points"] * steps / b["wall_s"]
     prof = b["prof"]
-    # h2 sweeps read and write the rows inside l-pairs (of each parity), before and after the solve
-    h2_bytes = sum(2 * 32 * c.r_points * (2 * (c.l_bound // 2) + 2 * ((c.l_bound - 1) // 2)) for c in compiled)
+    # h2 sweeps of k_sh_vphase read and write the rows inside l-pairs, before and after the solve:
+    # the even pairs always, the odd pairs only where they are not solved with the R-phase
+    fused = lambda c: 2 * c.P <= 512
+    h2_rows = lambda c: 2 * (c.l_bound // 2) + (0 if fused(c) else 2 * ((c.l_bound - 1) // 2))
+    h2_bytes = sum(2 * 32 * c.r_points * h2_rows(c) for c in compiled)
+    pts = sum(c.r_points * c.l_bound for c in compiled)
+    bytes_step = 64 * pts + h2_bytes  # L-phase and R-phase sweeps plus the separate h2 sweeps
     h2_ms = prof["vphase_ms"] / steps
     out = {
         "workload": f"sh_scan draws ({len(compiled)} simulations), velocity gauge, one hipGraph replay per step",
         "value": rate, "unit": "mesh-point-steps/s", "ms_per_step": b["wall_s"] * 1e3 / steps,
-        "bytes_per_point_step": 192, "step_hbm_gbs": rate * 192 / 1e9, "build_s": build_s,
+        "bytes_per_point_step": bytes_step / pts, "step_hbm_gbs": rate * (bytes_step / pts) / 1e9, "build_s": build_s,
         "kernels_ms_per_step": {"k_sh_lphase": prof["lphase_ms"] / steps, "k_sh_rphase": prof["rphase_ms"] / steps,
-                                "k_sh_vphase": h2_ms},
+                                "k_sh_rphase_vpair": prof["rpair_ms"] / steps, "k_sh_vphase": h2_ms},
         "k_sh_vphase_gbs": h2_bytes / (h2_ms * 1e-3) / 1e9 if h2_ms > 0 else None,
     }
     log(f"velocity gauge: {out['ms_per_step']:.3f} ms/step, {rate / 1e9:.2f} G mesh-point-steps/s; "
@@ -497,8 +505,13 @@ def summarize_velocity_profiles(d, command_trace, command_fetch, command_write, 
 
     fs, ws = cal(fetch), cal(write)
     csteps = counter_warmup + 2 * counter_steps
-    alg = sum(2 * 32 * p["r_points"] * (2 * (p["l_bound"] // 2) + 2 * ((p["l_bound"] - 1) // 2))
-              for p in workloads.sh_scan_params(sims, seed))  # h2 rows read + written per step
+    from ionization_amd import engine
+
+    def h2_rows(p):  # rows of k_sh_vphase sweeps: even l-pairs, and odd ones unless fused with the R-phase
+        P, _ = engine.choose_layout(p["r_points"], "sh")
+        return 2 * (p["l_bound"] // 2) + (0 if 2 * P <= 512 else 2 * ((p["l_bound"] - 1) // 2))
+
+    alg = sum(2 * 32 * p["r_points"] * h2_rows(p) for p in workloads.sh_scan_params(sims, seed))
     fb = sum(v for k, (c, v) in fetch.items() if _kernel_family(k) == "k_sh_vphase") * fs / csteps
     wb = sum(v for k, (c, v) in write.items() if _kernel_family(k) == "k_sh_vphase") * ws / csteps
     lv = [(k, c, v) for k, (c, v) in fetch.items() if k.startswith("void k_sh_lphase<false, true>")]
@@ -512,6 +525,14 @@ def summarize_velocity_profiles(d, command_trace, command_fetch, command_write, 
     if lv and lw:
         lines.append(f"k_sh_lphase<false, true> (velocity L-phase) per launch: fetch "
                      f"{lv[0][2] / lv[0][1] * fs / 1e6:.1f} MB, write {lw[0][2] / lw[0][1] * ws / 1e6:.1f} MB")
+    pf = sum(v for k, (c, v) in fetch.items() if _kernel_family(k) == "k_sh_rphase_vpair") * fs / csteps
+    pw = sum(v for k, (c, v) in write.items() if _kernel_family(k) == "k_sh_rphase_vpair") * ws / csteps
+    pair_alg = sum(16 * p["r_points"] * 2 * ((p["l_bound"] - 1) // 2) for p in workloads.sh_scan_params(sims, seed)
+                   if 2 * engine.choose_layout(p["r_points"], "sh")[0] <= 512)
+    if pair_alg:
+        lines.append(f"k_sh_rphase_vpair (odd l-pairs: R-phase with their h2 factors) per step: fetch {pf / 1e6:.1f} MB, "
+                     f"write {pw / 1e6:.1f} MB; algorithmic {pair_alg / 1e6:.1f} MB each way "
+                     f"({pf / pair_alg:.2f}x / {pw / pair_alg:.2f}x)")
     with open(os.path.join(prof_dir, "sh_velocity_scan_hbm_traffic.txt"), "w") as fh:
         fh.write("\n".join(lines) + "\n")
     print(open(os.path.join(prof_dir, "sh_velocity_scan_kernel_trace.txt")).read())

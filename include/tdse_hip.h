@@ -270,7 +270,7 @@ int tdse_profile(tdse_ctx *ctx, int32_t enable);
 int tdse_profile_read(tdse_ctx *ctx, double *lphase_ms, int64_t *lphase_launches, double *rphase_ms,
                       int64_t *rphase_launches);
 /* Same accumulators by kernel: 0 = SH L-phase, 1 = SH R-phase, 2 = Cyl z-sweep, 3 = Cyl rho-sweep,
- * 4 = SH velocity-gauge h2 factors. */
+ * 4 = SH velocity-gauge h2 factors, 5 = SH velocity-gauge odd l-pairs (R-phase with their h2 factors). */
 int tdse_profile_kernel(tdse_ctx *ctx, int32_t kernel, double *ms, int64_t *launches);
 
 #ifdef __cplusplus

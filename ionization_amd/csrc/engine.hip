@@ -30,6 +30,7 @@ constexpr int kGraphSteps = 8;
 constexpr int kMaxL = 4096;          // largest l_bound accepted (c_l table size)
 constexpr int kSlotCapacity = 8192;  // simulations of each kind per context
 constexpr int kSideStreams = 3;      // extra streams for concurrent R-phase buckets (default priority)
+constexpr int kMaxPairThreads = 512; // largest workgroup of the two-line (velocity odd l-pair) R-phase
 
 thread_local std::string g_lib_error;
 
@@ -141,12 +142,14 @@ struct Bucket {
     int lanes = 1;      // cylindrical rho-sweep: lines per workgroup
     DevVec<int2> lines;  // SH: (slot, l); Line: (slot, 0); Cyl z: (slot, rho); Cyl rho: (slot, position group)
     DevVec<int2> vpairs[2];  // SH: the velocity-gauge l-pairs (slot, l) of even / odd l among these simulations
+    DevVec<int2> fpairs;     // SH: velocity-gauge odd l-pairs solved with their h2 factors (k_sh_rphase_vpair)
 };
 
 void free_bucket(Bucket& b) {
     if (b.lines.ptr) (void)hipFree(b.lines.ptr);
     for (auto& v : b.vpairs)
         if (v.ptr) (void)hipFree(v.ptr);
+    if (b.fpairs.ptr) (void)hipFree(b.fpairs.ptr);
     b = Bucket{};
 }
 
@@ -198,10 +201,11 @@ struct tdse_ctx {
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
-    // (kind, start event index); kinds: SH L-phase, SH R-phase, Cyl z, Cyl rho, SH velocity-gauge h2 factors
+    // (kind, start event index); kinds: SH L-phase, SH R-phase, Cyl z, Cyl rho, SH velocity-gauge h2
+    // factors, SH velocity-gauge odd l-pair R-phase
     std::vector<std::pair<int, int>> ev_pairs;
-    double prof_ms[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    int64_t prof_n[5] = {0, 0, 0, 0, 0};
+    double prof_ms[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int64_t prof_n[6] = {0, 0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -250,6 +254,7 @@ int rebuild_tables(tdse_ctx* ctx) {
     ctx->n_len_waves = -1;
     std::map<std::pair<int, int>, std::vector<int2>> shb, lnb;  // (rows, chunks [+ 4096 if complex steps])
     std::map<std::pair<int, int>, std::vector<int2>> vpb[2];    // velocity l-pairs by parity, (rows, chunks)
+    std::map<std::pair<int, int>, std::vector<int2>> fpb;       // velocity odd l-pairs solved in one workgroup
     ctx->sh_norm_by_l = false;
     for (int pass = 0; pass < 2; ++pass)
     for (int s = 0; s < (int)ctx->sh.size(); ++s) {
@@ -272,9 +277,18 @@ int rebuild_tables(tdse_ctx* ctx) {
         }
         for (int b = 0; b < h.dev.n_basis; ++b) jobs.push_back(make_int2(s, b));
         auto& v = shb[{h.dev.m, h.dev.P}];
-        for (int l = 0; l < h.dev.L; ++l) v.push_back(make_int2(s, l));
+        // velocity gauge: where two lines fit one workgroup (2 P <= 512) the odd l-pairs are
+        // solved together with their h2 factors; the rows outside them stay single lines
+        const bool fuse = h.vel && 2 * h.dev.P <= kMaxPairThreads;
+        for (int l = 0; l < h.dev.L; ++l) {
+            const bool in_odd_pair = (l & 1) ? (l + 1 < h.dev.L) : (l >= 1);
+            if (!(fuse && in_odd_pair)) v.push_back(make_int2(s, l));
+        }
         if (h.vel)
-            for (int l = 0; l + 1 < h.dev.L; ++l) vpb[l & 1][{h.dev.m, h.dev.P}].push_back(make_int2(s, l));
+            for (int l = 0; l + 1 < h.dev.L; ++l) {
+                if (fuse && (l & 1)) fpb[{h.dev.m, h.dev.P}].push_back(make_int2(s, l));
+                else vpb[l & 1][{h.dev.m, h.dev.P}].push_back(make_int2(s, l));
+            }
     }
     if (ctx->n_len_waves < 0) ctx->n_len_waves = (int)waves.size();
     for (int s = 0; s < (int)ctx->line.size(); ++s) {
@@ -315,12 +329,16 @@ int rebuild_tables(tdse_ctx* ctx) {
         }
         return TDSE_OK;
     };
+    for (auto& kv : fpb) shb[kv.first];  // a bucket for every layout with fused pairs, even without single lines
     if ((rc = fill(ctx->sh_buckets, shb))) return rc;
-    for (auto& b : ctx->sh_buckets)
+    for (auto& b : ctx->sh_buckets) {
         for (int par = 0; par < 2; ++par) {
             auto it = vpb[par].find({b.M, b.P});
             if (it != vpb[par].end() && (rc = devvec_upload(ctx, b.vpairs[par], it->second))) return rc;
         }
+        auto it = fpb.find({b.M, b.P});
+        if (it != fpb.end() && (rc = devvec_upload(ctx, b.fpairs, it->second))) return rc;
+    }
     if ((rc = fill(ctx->line_buckets, lnb))) return rc;
     if ((rc = fill(ctx->cyl_z_buckets, czb))) return rc;
     if ((rc = fill(ctx->cyl_r_buckets, crb))) return rc;
@@ -330,7 +348,8 @@ int rebuild_tables(tdse_ctx* ctx) {
             // rows swept per step: the R-phase's lines, and the velocity-gauge h2 factors' rows
             // (two rows per pair, before and after the solve)
             const Bucket& b = ctx->sh_buckets[i];
-            const double rows = (double)b.lines.count + 4.0 * (double)(b.vpairs[0].count + b.vpairs[1].count);
+            const double rows = (double)b.lines.count + 2.0 * (double)b.fpairs.count +
+                                4.0 * (double)(b.vpairs[0].count + b.vpairs[1].count);
             w.push_back({-rows * b.P * b.M, (int)i});
         }
         std::sort(w.begin(), w.end());
@@ -406,6 +425,27 @@ void dispatch_rphase(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
         case 12: b.P > 256 ? launch_rphase<12, 512, 1>(ctx, b, st) : launch_rphase<12, 256, 2>(ctx, b, st); break;
         case 14: b.P > 256 ? launch_rphase<14, 512, 1>(ctx, b, st) : launch_rphase<14, 256, 2>(ctx, b, st); break;
         default: b.P > 256 ? launch_rphase<16, 512, 1>(ctx, b, st) : launch_rphase<16, 256, 1>(ctx, b, st); break;
+    }
+}
+
+template <int M, int WPE>
+void launch_rphase_vpair(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
+    const size_t lds = 3 * (size_t)b.P * 2 * sizeof(double2);
+    hipLaunchKernelGGL((k_sh_rphase_vpair<M, kMaxPairThreads, WPE>), dim3((unsigned)b.fpairs.count), dim3(2 * b.P),
+                       lds, st, ctx->d_sh, b.fpairs.ptr, (int)b.fpairs.count, ctx->d_cl);
+}
+
+// waves per EU: the most that leave the two-line kernel without spills up to m = 14
+void dispatch_rphase_vpair(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
+    switch (b.M) {
+        case 2: launch_rphase_vpair<2, 4>(ctx, b, st); break;
+        case 4: launch_rphase_vpair<4, 4>(ctx, b, st); break;
+        case 6: launch_rphase_vpair<6, 4>(ctx, b, st); break;
+        case 8: launch_rphase_vpair<8, 3>(ctx, b, st); break;
+        case 10: launch_rphase_vpair<10, 2>(ctx, b, st); break;
+        case 12: launch_rphase_vpair<12, 2>(ctx, b, st); break;
+        case 14: launch_rphase_vpair<14, 2>(ctx, b, st); break;
+        default: launch_rphase_vpair<16, 1>(ctx, b, st); break;
     }
 }
 
@@ -541,9 +581,16 @@ int launch_step(tdse_ctx* ctx, bool cyl_pre) {
                     dispatch_vphase<false>(ctx, b, par, st);
                     prof_end(ctx);
                 }
-            prof_begin(ctx, 1);
-            dispatch_rphase(ctx, b, st);
-            prof_end(ctx);
+            if (b.lines.count) {
+                prof_begin(ctx, 1);
+                dispatch_rphase(ctx, b, st);
+                prof_end(ctx);
+            }
+            if (b.fpairs.count) {
+                prof_begin(ctx, 5);
+                dispatch_rphase_vpair(ctx, b, st);
+                prof_end(ctx);
+            }
             for (int par = 1; par >= 0; --par)
                 if (b.vpairs[par].count) {
                     prof_begin(ctx, 4);
@@ -1414,7 +1461,7 @@ int tdse_last_run_ms(tdse_ctx* ctx, double* ms) {
 int tdse_profile(tdse_ctx* ctx, int32_t enable) {
     if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
     ctx->profiling = enable != 0;
-    for (int i = 0; i < 5; ++i) {
+    for (int i = 0; i < 6; ++i) {
         ctx->prof_ms[i] = 0.0;
         ctx->prof_n[i] = 0;
     }
@@ -1433,7 +1480,7 @@ int tdse_profile_read(tdse_ctx* ctx, double* lphase_ms, int64_t* lphase_launches
 
 int tdse_profile_kernel(tdse_ctx* ctx, int32_t kernel, double* ms, int64_t* launches) {
     if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
-    if (kernel < 0 || kernel > 4) return fail(ctx, TDSE_ERR_ARG, "kernel must be 0..4");
+    if (kernel < 0 || kernel > 5) return fail(ctx, TDSE_ERR_ARG, "kernel must be 0..5");
     if (ms) *ms = ctx->prof_ms[kernel];
     if (launches) *launches = ctx->prof_n[kernel];
     return TDSE_OK;

@@ -572,6 +572,118 @@ __global__ void __launch_bounds__(512) k_sh_vphase(ShSim* __restrict__ sims, con
     }
 }
 
+// Velocity gauge: the R-phase of one odd l-pair (l, l + 1) together with the
+// pair's h2 similarity factors (OE, OO before the CN solve; OO, OE after it), so
+// the odd pairs' factors cost no sweep of their own. The two rows are
+// interleaved over the workgroup: thread t works on row l + (t & 1) and chunk
+// k = t >> 1, so the partner value of the pair transform sits in the
+// neighbouring lane (one lane swap, no LDS); the CN solve is the R-phase's,
+// with two lines per workgroup (LANES = 2 in LDS). Both rows have l >= 1: no
+// l = 0 correction.
+template <int M>
+__device__ __forceinline__ void sh_vpair_factors(double2 (&x)[M], const int k, const int P, const int ln,
+                                                 double2* __restrict__ lds, const double* __restrict__ av,
+                                                 const double fsg, const bool rev) {
+    constexpr double kRsqrt2 = 0.70710678118654752440;
+    // to the pair basis and back: row l keeps (own + partner) / sqrt 2, row l + 1 (partner - own) / sqrt 2
+    auto hadamard = [&]() {
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const double2 o = cmk(__shfl_xor(x[i].x, 1), __shfl_xor(x[i].y, 1));
+            x[i] = ln ? cmk((o.x - x[i].x) * kRsqrt2, (o.y - x[i].y) * kRsqrt2)
+                      : cmk((x[i].x + o.x) * kRsqrt2, (x[i].y + o.y) * kRsqrt2);
+        }
+    };
+    auto even = [&]() {
+#pragma unroll
+        for (int i = 0; i < M; i += 2) sh_rotate<true>(fsg * ld1(av, (size_t)i * P + k), x[i], x[i + 1]);
+    };
+    auto odd = [&]() {
+#pragma unroll
+        for (int i = 1; i + 1 < M; i += 2) sh_rotate<true>(fsg * ld1(av, (size_t)i * P + k), x[i], x[i + 1]);
+        __syncthreads();  // the LDS rows may still be read by the solver
+        lds[k * 2 + ln] = x[M - 1];
+        lds[(P + k) * 2 + ln] = x[0];
+        __syncthreads();
+        double2 r = (k + 1 < P) ? lds[(P + k + 1) * 2 + ln] : cmk(0.0, 0.0);
+        const double thr = (k + 1 < P) ? fsg * ld1(av, (size_t)(M - 1) * P + k) : 0.0;
+        sh_rotate<true>(thr, x[M - 1], r);
+        double2 lf = (k > 0) ? lds[(k - 1) * 2 + ln] : cmk(0.0, 0.0);
+        const double thl = (k > 0) ? fsg * ld1(av, (size_t)(M - 1) * P + k - 1) : 0.0;
+        sh_rotate<true>(thl, lf, x[0]);
+    };
+    hadamard();
+    if (rev) {
+        odd();
+        even();
+    } else {
+        even();
+        odd();
+    }
+    hadamard();
+}
+
+template <int M, int MAXT, int WPE>
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+k_sh_rphase_vpair(ShSim* __restrict__ sims, const int2* __restrict__ pairs, int n_pairs,
+                  const double* __restrict__ cl) {
+    extern __shared__ double2 lds[];  // 3 * P * 2: the solver's rows of both lines, interleaved
+    const int job = xcd_remap(blockIdx.x, gridDim.x);
+    if (job >= n_pairs) return;
+    const int2 pt = pairs[job];
+    ShSim& S = sims[pt.x];
+    const int n = S.step;
+    if (!S.active || n >= S.N) return;  // uniform across the block
+    const int P = S.P, npad = S.npad;
+    const int ln = threadIdx.x & 1, k = threadIdx.x >> 1;
+    const int l = pt.y + ln;
+    const double sn = ld1(S.ts, n);
+    const double ll1 = (double)l * (double)(l + 1);
+    double2* __restrict__ gl = sh_state(S, n + 1) + (size_t)l * npad + k;
+    const double2* __restrict__ TA = S.ta + k;
+    const double dr = S.delta_r, cu = S.cu;
+    const int nr = S.n;
+    const double a_left = (k > 0) ? ld1(S.tab, SH_AN * npad + (M - 1) * P + k - 1) : 0.0;
+    const double fh = ld1(S.fs, n);
+    const double fsg = (ln ? -fh : fh) * ld1(cl, pt.y);  // +theta on row l's combination, -theta on row l + 1's
+    const double* __restrict__ av = S.tab + (size_t)SH_AV * npad;
+
+    double2 x[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) x[i] = ld2s(gl, (size_t)i * P);
+    if (fh != 0.0) sh_vpair_factors<M>(x, k, P, ln, lds, av, fsg, false);
+
+    auto diag = [&](int i, double T) {
+        const int j = k * M + i;
+        const double r = fma((double)j, dr, 0.5 * dr);
+        const double U = (j < nr) ? cu * tdse_rcp(r * r) : 0.0;
+        return sn * fma(ll1, U, T);
+    };
+    double carry = a_left;
+    auto coef = [&](int i, double& hm, double& h, double& hp) {
+        const double2 tv = ld2(TA, (size_t)i * P);
+        h = diag(i, tv.x);
+        hm = sn * carry;
+        hp = sn * tv.y;
+        carry = tv.y;
+    };
+    double carry2 = a_left;
+    auto coef2 = [&](int i, double& hm, double& h) {
+        const double2* TA2 = TA;
+        asm volatile("" : "+v"(TA2));
+        const double2 tv = ld2(TA2, (size_t)i * P);
+        h = diag(i, tv.x);
+        hm = sn * carry2;
+        carry2 = tv.y;
+    };
+    double2 xl, xr;
+    chunk_neighbours<M, 2>(x, k, P, lds, xl, xr, true, ln);
+    cn_implicit_lowreg<M, decltype(coef), decltype(coef2), true, 2>(x, k, P, coef, coef2, lds, xl, xr, ln);
+    if (fh != 0.0) sh_vpair_factors<M>(x, k, P, ln, lds, av, fsg, true);
+#pragma unroll
+    for (int i = 0; i < M; ++i) st2s(gl, (size_t)i * P, x[i]);
+}
+
 // Advance every active simulation's step counter after a full graph step.
 __global__ void k_sh_advance(ShSim* sims, int n_sims) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;

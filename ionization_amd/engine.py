@@ -802,7 +802,7 @@ class DeviceContext:
 
     def profile_kernel(self, kernel: int):
         """(ms, launches) accumulated for kernel 0 = SH L-phase, 1 = SH R-phase, 2 = Cyl z, 3 = Cyl rho,
-        4 = SH velocity-gauge h2 factors."""
+        4 = SH velocity-gauge h2 factors, 5 = SH velocity-gauge odd l-pairs (R-phase with their h2 factors)."""
         ms, n = ctypes.c_double(), ctypes.c_int64()
         self._check(self._lib.tdse_profile_kernel(self._ctx, int(kernel), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
