@@ -580,11 +580,15 @@ __global__ void __launch_bounds__(512) k_sh_vphase(ShSim* __restrict__ sims, con
 // neighbouring lane (one lane swap, no LDS); the CN solve is the R-phase's,
 // with two lines per workgroup (LANES = 2 in LDS). Both rows have l >= 1: no
 // l = 0 correction.
+// The two lanes of a chunk (rows l and l + 1 of the pair) turn by +theta and
+// -theta on the same radial pairs: each lane computes the sine and cosine of
+// every other pair's angle and takes the rest from its neighbour lane.
 template <int M>
 __device__ __forceinline__ void sh_vpair_factors(double2 (&x)[M], const int k, const int P, const int ln,
                                                  double2* __restrict__ lds, const double* __restrict__ av,
-                                                 const double fsg, const bool rev) {
+                                                 const double f, const bool rev) {
     constexpr double kRsqrt2 = 0.70710678118654752440;
+    const double sg = ln ? -1.0 : 1.0;
     // to the pair basis and back: row l keeps (own + partner) / sqrt 2, row l + 1 (partner - own) / sqrt 2
     auto hadamard = [&]() {
 #pragma unroll
@@ -594,30 +598,62 @@ __device__ __forceinline__ void sh_vpair_factors(double2 (&x)[M], const int k, c
                       : cmk((x[i].x + o.x) * kRsqrt2, (x[i].y + o.y) * kRsqrt2);
         }
     };
-    auto even = [&]() {
+    // the real rotation by sg * theta, from the sine and cosine of theta
+    auto rot = [&](double sn, const double cs, double2& u, double2& v) {
+        sn *= sg;
+        const double2 y0 = cmk(fma(cs, u.x, sn * v.x), fma(cs, u.y, sn * v.y));
+        const double2 y1 = cmk(fma(cs, v.x, -sn * u.x), fma(cs, v.y, -sn * u.y));
+        u = y0;
+        v = y1;
+    };
+    // sine and cosine of lane 0's angle ta and lane 1's angle tb, on both lanes
+    auto sincos2 = [&](const double ta, const double tb, double& sa, double& ca, double& sb, double& cb) {
+        const double th = ln ? tb : ta;
+        double sm, cm;
+        if (__all(fabs(th) <= kSmallAngle)) tdse_sincos_small(th, &sm, &cm);
+        else tdse_sincos(th, &sm, &cm);
+        const double so = __shfl_xor(sm, 1), co = __shfl_xor(cm, 1);
+        sa = ln ? so : sm;
+        ca = ln ? co : cm;
+        sb = ln ? sm : so;
+        cb = ln ? cm : co;
+    };
+    auto th = [&](int i) { return f * ld1(av, (size_t)i * P + k); };
+    // radial pairs (i, i + 1), i = first, first + 2, ... < M - 1, two per sine/cosine evaluation
+    auto pairs = [&](const int first) {
 #pragma unroll
-        for (int i = 0; i < M; i += 2) sh_rotate<true>(fsg * ld1(av, (size_t)i * P + k), x[i], x[i + 1]);
+        for (int i = first; i + 1 < M; i += 4) {
+            const int i2 = i + 2;
+            const double ta = th(i);
+            const double tb = (i2 + 1 < M) ? th(i2) : ta;
+            double sa, ca, sb, cb;
+            sincos2(ta, tb, sa, ca, sb, cb);
+            rot(sa, ca, x[i], x[i + 1]);
+            if (i2 + 1 < M) rot(sb, cb, x[i2], x[i2 + 1]);
+        }
     };
     auto odd = [&]() {
-#pragma unroll
-        for (int i = 1; i + 1 < M; i += 2) sh_rotate<true>(fsg * ld1(av, (size_t)i * P + k), x[i], x[i + 1]);
+        pairs(1);
         __syncthreads();  // the LDS rows may still be read by the solver
         lds[k * 2 + ln] = x[M - 1];
         lds[(P + k) * 2 + ln] = x[0];
         __syncthreads();
+        // the chunk-edge pairs: to the right (my row M - 1, the next chunk's row 0) and to the left
         double2 r = (k + 1 < P) ? lds[(P + k + 1) * 2 + ln] : cmk(0.0, 0.0);
-        const double thr = (k + 1 < P) ? fsg * ld1(av, (size_t)(M - 1) * P + k) : 0.0;
-        sh_rotate<true>(thr, x[M - 1], r);
         double2 lf = (k > 0) ? lds[(k - 1) * 2 + ln] : cmk(0.0, 0.0);
-        const double thl = (k > 0) ? fsg * ld1(av, (size_t)(M - 1) * P + k - 1) : 0.0;
-        sh_rotate<true>(thl, lf, x[0]);
+        const double thr = (k + 1 < P) ? th(M - 1) : 0.0;
+        const double thl = (k > 0) ? f * ld1(av, (size_t)(M - 1) * P + k - 1) : 0.0;
+        double sr, cr, sl, cl;
+        sincos2(thr, thl, sr, cr, sl, cl);
+        rot(sr, cr, x[M - 1], r);
+        rot(sl, cl, lf, x[0]);
     };
     hadamard();
     if (rev) {
         odd();
-        even();
+        pairs(0);
     } else {
-        even();
+        pairs(0);
         odd();
     }
     hadamard();
@@ -645,13 +681,13 @@ k_sh_rphase_vpair(ShSim* __restrict__ sims, const int2* __restrict__ pairs, int 
     const int nr = S.n;
     const double a_left = (k > 0) ? ld1(S.tab, SH_AN * npad + (M - 1) * P + k - 1) : 0.0;
     const double fh = ld1(S.fs, n);
-    const double fsg = (ln ? -fh : fh) * ld1(cl, pt.y);  // +theta on row l's combination, -theta on row l + 1's
+    const double fp = fh * ld1(cl, pt.y);  // the pair's angle factor (row l's combination +theta, row l + 1's -theta)
     const double* __restrict__ av = S.tab + (size_t)SH_AV * npad;
 
     double2 x[M];
 #pragma unroll
     for (int i = 0; i < M; ++i) x[i] = ld2s(gl, (size_t)i * P);
-    if (fh != 0.0) sh_vpair_factors<M>(x, k, P, ln, lds, av, fsg, false);
+    if (fh != 0.0) sh_vpair_factors<M>(x, k, P, ln, lds, av, fp, false);
 
     auto diag = [&](int i, double T) {
         const int j = k * M + i;
@@ -679,7 +715,7 @@ k_sh_rphase_vpair(ShSim* __restrict__ sims, const int2* __restrict__ pairs, int 
     double2 xl, xr;
     chunk_neighbours<M, 2>(x, k, P, lds, xl, xr, true, ln);
     cn_implicit_lowreg<M, decltype(coef), decltype(coef2), true, 2>(x, k, P, coef, coef2, lds, xl, xr, ln);
-    if (fh != 0.0) sh_vpair_factors<M>(x, k, P, ln, lds, av, fsg, true);
+    if (fh != 0.0) sh_vpair_factors<M>(x, k, P, ln, lds, av, fp, true);
 #pragma unroll
     for (int i = 0; i < M; ++i) st2s(gl, (size_t)i * P, x[i]);
 }
