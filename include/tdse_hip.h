@@ -100,6 +100,9 @@ typedef struct {
     int32_t norm_by_l;     /* 1: also reduce the norm in every l at data times (NormBySphericalHarmonic) */
     int32_t gauge;         /* 0: length gauge; 1: velocity gauge (SphericalHarmonicVelocityGaugeOperators,
                               mesh_operators.py:1130-1408): r_points >= 3, l_bound >= 2 */
+    int32_t method;        /* 0: SplitInteractionOperator; 1: AlternatingDirectionImplicit (length gauge;
+                              evolution_methods.py:46-77 with the total Hamiltonian of mesh_operators.py:1020-1035) */
+    int32_t reserved;
 } tdse_sh_desc;
 
 /*

@@ -34,6 +34,8 @@ class ShDesc(ctypes.Structure):
         ("centrifugal", ctypes.c_double),
         ("norm_by_l", ctypes.c_int32),
         ("gauge", ctypes.c_int32),
+        ("method", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

@@ -104,6 +104,7 @@ class Arena {
 struct ShHost {
     bool live = false;
     bool vel = false;    // velocity gauge (one more table row, h2 factors around the R-phase)
+    bool adi = false;    // AlternatingDirectionImplicit (its own r and l sweeps instead of the R-phase)
     ShSim dev{};
     int n_data = 0;
     int wave_first = 0;  // this simulation's range in the wave table
@@ -185,6 +186,8 @@ struct tdse_ctx {
     DevVec<int2> jobs;
     std::vector<Bucket> sh_buckets;
     std::vector<int> sh_order;  // sh_buckets by decreasing work
+    std::vector<Bucket> sh_adi_buckets;  // AlternatingDirectionImplicit l-lines (slot, l) by (rows, chunks)
+    DevVec<int2> adi_waves;              // AlternatingDirectionImplicit positions: (slot, first position of a wave)
     hipStream_t side[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
     std::vector<Bucket> line_buckets;
@@ -255,6 +258,8 @@ int rebuild_tables(tdse_ctx* ctx) {
     std::map<std::pair<int, int>, std::vector<int2>> shb, lnb;  // (rows, chunks [+ 4096 if complex steps])
     std::map<std::pair<int, int>, std::vector<int2>> vpb[2];    // velocity l-pairs by parity, (rows, chunks)
     std::map<std::pair<int, int>, std::vector<int2>> fpb;       // velocity odd l-pairs solved in one workgroup
+    std::map<std::pair<int, int>, std::vector<int2>> adb;       // ADI l-lines, (rows, chunks)
+    std::vector<int2> adw;                                      // ADI position waves
     ctx->sh_norm_by_l = false;
     for (int pass = 0; pass < 2; ++pass)
     for (int s = 0; s < (int)ctx->sh.size(); ++s) {
@@ -276,6 +281,11 @@ int rebuild_tables(tdse_ctx* ctx) {
             ctx->sh_norm_by_l = true;
         }
         for (int b = 0; b < h.dev.n_basis; ++b) jobs.push_back(make_int2(s, b));
+        if (h.adi) {
+            for (int l = 0; l < h.dev.L; ++l) adb[{h.dev.m, h.dev.P}].push_back(make_int2(s, l));
+            for (int w = 0; w < wpl; ++w) adw.push_back(make_int2(s, w * TDSE_WAVE));
+            continue;
+        }
         auto& v = shb[{h.dev.m, h.dev.P}];
         // velocity gauge: where two lines fit one workgroup (2 P <= 512) the odd l-pairs are
         // solved together with their h2 factors; the rows outside them stay single lines
@@ -331,6 +341,8 @@ int rebuild_tables(tdse_ctx* ctx) {
     };
     for (auto& kv : fpb) shb[kv.first];  // a bucket for every layout with fused pairs, even without single lines
     if ((rc = fill(ctx->sh_buckets, shb))) return rc;
+    if ((rc = fill(ctx->sh_adi_buckets, adb))) return rc;
+    if ((rc = devvec_upload(ctx, ctx->adi_waves, adw))) return rc;
     for (auto& b : ctx->sh_buckets) {
         for (int par = 0; par < 2; ++par) {
             auto it = vpb[par].find({b.M, b.P});
@@ -425,6 +437,28 @@ void dispatch_rphase(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
         case 12: b.P > 256 ? launch_rphase<12, 512, 1>(ctx, b, st) : launch_rphase<12, 256, 2>(ctx, b, st); break;
         case 14: b.P > 256 ? launch_rphase<14, 512, 1>(ctx, b, st) : launch_rphase<14, 256, 2>(ctx, b, st); break;
         default: b.P > 256 ? launch_rphase<16, 512, 1>(ctx, b, st) : launch_rphase<16, 256, 1>(ctx, b, st); break;
+    }
+}
+
+template <int M, int MAXT, int WPE, bool EXPL>
+void launch_adi_r(tdse_ctx* ctx, const Bucket& b) {
+    const size_t lds = 3 * (size_t)b.P * sizeof(double2);
+    hipLaunchKernelGGL((k_sh_adi_r<M, MAXT, WPE, EXPL>), dim3((unsigned)b.lines.count), dim3(b.P), lds, ctx->stream,
+                       ctx->d_sh, b.lines.ptr, (int)b.lines.count);
+}
+
+// the R-phase's launch shapes
+template <bool EXPL>
+void dispatch_adi_r(tdse_ctx* ctx, const Bucket& b) {
+    switch (b.M) {
+        case 2: launch_adi_r<2, 512, 4, EXPL>(ctx, b); break;
+        case 4: launch_adi_r<4, 512, 4, EXPL>(ctx, b); break;
+        case 6: launch_adi_r<6, 512, 4, EXPL>(ctx, b); break;
+        case 8: launch_adi_r<8, 512, 4, EXPL>(ctx, b); break;
+        case 10: b.P > 256 ? launch_adi_r<10, 512, 1, EXPL>(ctx, b) : launch_adi_r<10, 256, 3, EXPL>(ctx, b); break;
+        case 12: b.P > 256 ? launch_adi_r<12, 512, 1, EXPL>(ctx, b) : launch_adi_r<12, 256, 2, EXPL>(ctx, b); break;
+        case 14: b.P > 256 ? launch_adi_r<14, 512, 1, EXPL>(ctx, b) : launch_adi_r<14, 256, 2, EXPL>(ctx, b); break;
+        default: b.P > 256 ? launch_adi_r<16, 512, 1, EXPL>(ctx, b) : launch_adi_r<16, 256, 1, EXPL>(ctx, b); break;
     }
 }
 
@@ -615,6 +649,14 @@ int launch_step(tdse_ctx* ctx, bool cyl_pre) {
                 HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join[j], 0));
             }
         }
+        // AlternatingDirectionImplicit simulations: explicit H0, the l-direction factor, implicit H0
+        if (!ctx->sh_adi_buckets.empty()) {
+            for (auto& b : ctx->sh_adi_buckets) dispatch_adi_r<true>(ctx, b);
+            const int na = (int)ctx->adi_waves.count;
+            hipLaunchKernelGGL(k_sh_adi_l, dim3((na * TDSE_WAVE + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh,
+                               ctx->adi_waves.ptr, na, ctx->d_cl);
+            for (auto& b : ctx->sh_adi_buckets) dispatch_adi_r<false>(ctx, b);
+        }
         const int ns = (int)ctx->sh.size();
         hipLaunchKernelGGL(k_sh_advance, dim3((ns + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh, ns);
     }
@@ -760,8 +802,10 @@ int tdse_ctx_destroy(tdse_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     drop_graph(ctx);
-    for (auto* v : {&ctx->sh_buckets, &ctx->line_buckets, &ctx->cyl_z_buckets, &ctx->cyl_r_buckets})
+    for (auto* v : {&ctx->sh_buckets, &ctx->sh_adi_buckets, &ctx->line_buckets, &ctx->cyl_z_buckets,
+                    &ctx->cyl_r_buckets})
         for (auto& b : *v) free_bucket(b);
+    if (ctx->adi_waves.ptr) (void)hipFree(ctx->adi_waves.ptr);
     if (ctx->cyl_jobs.ptr) (void)hipFree(ctx->cyl_jobs.ptr);
     if (ctx->d_cyl) (void)hipFree(ctx->d_cyl);
     if (ctx->waves.ptr) (void)hipFree(ctx->waves.ptr);
@@ -892,6 +936,9 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
     if (D.gauge != 0 && D.gauge != 1) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: gauge must be 0 (length) or 1 (velocity)");
     if (D.gauge == 1 && (D.r_points < 3 || D.l_bound < 2))
         return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: the velocity gauge needs r_points >= 3 and l_bound >= 2");
+    if (D.method != 0 && D.method != 1) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: method must be 0 or 1");
+    if (D.method == 1 && D.gauge != 0)
+        return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: AlternatingDirectionImplicit runs the length gauge");
     for (int b = 0; b < D.n_basis; ++b)
         if (basis_l[b] < 0 || basis_l[b] >= D.lb) return fail(ctx, TDSE_ERR_ARG, "tdse_sh_add: basis l must be < lb");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -909,6 +956,7 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
     ShHost& H = ctx->sh[s];
     H.blocks.clear();
     H.vel = D.gauge == 1;
+    H.adi = D.method == 1;
     ShSim& S = H.dev;
     S = ShSim{};
     S.n = D.r_points;
@@ -930,7 +978,7 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
     H.n_data = n_data_points(S.N, S.store_every);
     const size_t npad = S.npad;
     double2 *g, *g2, *snap, *oip, *ta;
-    double *tab, *dfs, *dts, *part, *on, *oln, *bas = nullptr, *pl = nullptr, *onl = nullptr;
+    double *tab, *dfs, *dts, *part, *on, *oln, *bas = nullptr, *pl = nullptr, *onl = nullptr, *dfa = nullptr;
     int* bl = nullptr;
     int rc = 0;
     std::vector<double2> ta_host(npad);  // (T_j, A_j) interleaved for the R-phase
@@ -939,7 +987,9 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
         (rc = copy_to_device(ctx, H.blocks, ta_host.data(), npad, &ta)) ||
         (rc = copy_to_device<double2>(ctx, H.blocks, nullptr, (size_t)S.L * npad, &g2)) ||
         (rc = copy_to_device(ctx, H.blocks, tables, (SH_NTAB + (H.vel ? 1 : 0)) * npad, &tab)) ||
-        (rc = copy_to_device(ctx, H.blocks, fs, (size_t)S.N, &dfs)) ||
+        // ADI: the L-phase sees a zero field (it only masks and observes); the field goes to fa
+        (rc = copy_to_device(ctx, H.blocks, H.adi ? nullptr : fs, (size_t)S.N, &dfs)) ||
+        (H.adi && (rc = copy_to_device(ctx, H.blocks, fs, (size_t)S.N, &dfa))) ||
         (rc = copy_to_device(ctx, H.blocks, ts, (size_t)S.N, &dts)) ||
         (rc = copy_to_device<double2>(ctx, H.blocks, nullptr, (size_t)S.LB * npad, &snap)) ||
         (rc = copy_to_device<double>(ctx, H.blocks, nullptr, 2 * (size_t)S.nwaves, &part)) ||
@@ -970,6 +1020,8 @@ int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, c
     S.basis_l = bl;
     S.part_l = pl;
     S.out_nl = onl;
+    S.fa = dfa;
+    S.adi = H.adi ? 1 : 0;
     HIP_TRY(ctx, hipMemcpyAsync(ctx->d_sh + s, &S, sizeof(ShSim), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // host buffers may be released after return
     H.live = true;

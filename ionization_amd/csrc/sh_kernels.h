@@ -51,6 +51,8 @@ struct ShSim {
     int active;
     int phase;             // 0: g is mid-step (after the R-phase); 1: g is the reference g after `step` steps
     int norm_by_l;         // 1: also reduce the norm of every l at data times
+    int adi;               // 1: AlternatingDirectionImplicit (fs is then zero: the L-phase only masks and observes)
+    const double* fa;      // ADI: [N] field times tau ratio per step, E(t_{n+1} + dt/2) tau_n / tau0
 };
 
 constexpr int kShSegRows = 64;  // most rows of one L-phase wave (its l-segment)
@@ -718,6 +720,143 @@ k_sh_rphase_vpair(ShSim* __restrict__ sims, const int2* __restrict__ pairs, int 
     if (fh != 0.0) sh_vpair_factors<M>(x, k, P, ln, lds, av, fp, true);
 #pragma unroll
     for (int i = 0; i < M; ++i) st2s(gl, (size_t)i * P, x[i]);
+}
+
+// ------------------------------------------------------------------ AlternatingDirectionImplicit
+//
+// SphericalHarmonicMesh with AlternatingDirectionImplicit
+// (evolution_methods.py:46-77): the total Hamiltonian is [H0 along r,
+// E H_int along l] (mesh_operators.py:1020-1035), so one reference step is
+//     g <- M . (1 + i tau H0)^-1 . (1 - i tau H_int)(1 + i tau H_int)^-1 . (1 - i tau H0) g
+// with the field at t_{n+1} + dt/2 (mesh_operators.py:1008-1018). On the device
+// the L-phase (with zero rotation angles) applies the mask and observes at data
+// times, then three in-place sweeps of sh_state(n + 1) follow: k_sh_adi_r
+// (explicit H0), k_sh_adi_l (the l-direction Cayley factor), k_sh_adi_r
+// (implicit H0).
+
+// H0 along r for one l-line (the R-phase's layout and coefficients):
+// EXPL: y = (1 - i H~) x; otherwise y = (1 + i H~)^-1 x.
+template <int M, int MAXT, int WPE, bool EXPL>
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+k_sh_adi_r(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines) {
+    extern __shared__ double2 lds[];  // 3 * P
+    const int line = xcd_remap(blockIdx.x, gridDim.x);
+    if (line >= n_lines) return;
+    const int2 lt = lines[line];
+    ShSim& S = sims[lt.x];
+    const int n = S.step;
+    if (!S.active || n >= S.N) return;  // uniform across the block
+    const int l = lt.y;
+    const int P = S.P, npad = S.npad;
+    const int k = threadIdx.x;
+    const double sn = ld1(S.ts, n);
+    const double ll1 = (double)l * (double)(l + 1);
+    const double c0 = (l == 0 && k == 0) ? S.corr0 : 0.0;
+    double2* __restrict__ gl = sh_state(S, n + 1) + (size_t)l * npad + k;
+    const double2* __restrict__ TA = S.ta + k;
+    const double dr = S.delta_r, cu = S.cu;
+    const int nr = S.n;
+    const double a_left = (k > 0) ? ld1(S.tab, SH_AN * npad + (M - 1) * P + k - 1) : 0.0;
+
+    double2 x[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) x[i] = ld2s(gl, (size_t)i * P);
+    auto diag = [&](int i, double T) {
+        const int j = k * M + i;
+        const double r = fma((double)j, dr, 0.5 * dr);
+        const double U = (j < nr) ? cu * tdse_rcp(r * r) : 0.0;
+        return sn * (fma(ll1, U, T) + (i == 0 ? c0 : 0.0));
+    };
+    double carry = a_left;
+    auto coef = [&](int i, double& hm, double& h, double& hp) {
+        const double2 tv = ld2(TA, (size_t)i * P);
+        h = diag(i, tv.x);
+        hm = sn * carry;
+        hp = sn * tv.y;
+        carry = tv.y;
+    };
+    if constexpr (EXPL) {
+        double2 xl, xr;
+        chunk_neighbours<M>(x, k, P, lds, xl, xr, false);
+        double2 prev = xl;
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            double hm, h, hp;
+            coef(i, hm, h, hp);
+            const double2 xp = (i + 1 < M) ? x[i + 1 < M ? i + 1 : i] : xr;
+            const double2 hx = cmk(fma(hm, prev.x, fma(h, x[i].x, hp * xp.x)), fma(hm, prev.y, fma(h, x[i].y, hp * xp.y)));
+            prev = x[i];
+            x[i] = cmk(x[i].x + hx.y, x[i].y - hx.x);  // x - i H~ x
+        }
+    } else {
+        double carry2 = a_left;
+        auto coef2 = [&](int i, double& hm, double& h) {
+            const double2* TA2 = TA;
+            asm volatile("" : "+v"(TA2));
+            const double2 tv = ld2(TA2, (size_t)i * P);
+            h = diag(i, tv.x);
+            hm = sn * carry2;
+            carry2 = tv.y;
+        };
+        cn_implicit_lowreg<M, decltype(coef), decltype(coef2), false, 1>(x, k, P, coef, coef2, lds);
+    }
+#pragma unroll
+    for (int i = 0; i < M; ++i) st2s(gl, (size_t)i * P, x[i]);
+}
+
+// The l-direction factor (1 - i A)(1 + i A)^-1 at every position: A real
+// symmetric tridiagonal along l with zero diagonal and off-diagonals
+// a_l = tau_n E (-q) r_j c_l (zero from l = L - 1 on). One lane per position
+// (64 consecutive positions per wave, unit-stride rows), the Thomas recurrence
+// along l on the right-hand side (1 - i A) x: the pivots of 1 + i A are real,
+// den_l = 1 + a_{l-1} q_{l-1} >= 1 with q_l = a_l / den_l, so each row keeps one
+// real q_l. The forward pass stores d'_l in place and q_l in the other state
+// buffer (free between the L-phase and the next step's), the backward pass
+// y_l = d'_l - i q_l y_{l+1} reads them back. Padding positions are left alone.
+__global__ void __launch_bounds__(256) k_sh_adi_l(ShSim* __restrict__ sims, const int2* __restrict__ waves,
+                                                  int n_waves, const double* __restrict__ cl) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / TDSE_WAVE;
+    const int lane = threadIdx.x % TDSE_WAVE;
+    if (wave >= n_waves) return;
+    const int2 wt = waves[wave];
+    ShSim& S = sims[wt.x];
+    const int n = S.step;
+    if (!S.active || n >= S.N) return;
+    const int L = S.L, P = S.P, m = S.m, npad = S.npad;
+    const int p = wt.y + lane;
+    const int j = (p % P) * m + p / P;
+    if (j >= S.n) return;  // no barriers below: lanes may leave
+    const double f = ld1(S.fa, n) * ld1(S.tab, SH_RQ * npad + p);
+    double2* __restrict__ g = sh_state(S, n + 1) + p;
+    double* __restrict__ q = reinterpret_cast<double*>(sh_state(S, n)) + p;
+    double2 xm = cmk(0.0, 0.0), x0 = ld2(g, 0), dprev = cmk(0.0, 0.0);
+    double am = 0.0, qprev = 0.0;
+    for (int l = 0; l < L; ++l) {
+        const bool more = l + 1 < L;
+        const double a = more ? f * ld1(cl, l) : 0.0;
+        const double2 xp = more ? ld2(g, (size_t)(l + 1) * npad) : cmk(0.0, 0.0);
+        // right-hand side x - i (a_{l-1} x_{l-1} + a_l x_{l+1})
+        const double hx = fma(am, xm.x, a * xp.x), hy = fma(am, xm.y, a * xp.y);
+        const double2 rhs = cmk(x0.x + hy, x0.y - hx);
+        const double rden = tdse_rcp(fma(am, qprev, 1.0));
+        // d'_l = (rhs - i a_{l-1} d'_{l-1}) / den_l
+        const double2 d = cmk(fma(am, dprev.y, rhs.x) * rden, fma(-am, dprev.x, rhs.y) * rden);
+        const double ql = a * rden;
+        st2(g, (size_t)l * npad, d);
+        q[(size_t)l * npad] = ql;
+        xm = x0;
+        x0 = xp;
+        am = a;
+        dprev = d;
+        qprev = ql;
+    }
+    double2 y = dprev;  // y_{L-1} = d'_{L-1}, already in place
+    for (int l = L - 2; l >= 0; --l) {
+        const double2 d = ld2(g, (size_t)l * npad);
+        const double ql = ld1(q, (size_t)l * npad);
+        y = cmk(fma(ql, y.y, d.x), fma(-ql, y.x, d.y));  // d'_l - i q_l y_{l+1}
+        st2(g, (size_t)l * npad, y);
+    }
 }
 
 // Advance every active simulation's step counter after a full graph step.

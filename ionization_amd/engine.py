@@ -172,6 +172,7 @@ class CompiledSH:
     basis_l: np.ndarray  # (nb,) int32
     norm_by_l: bool = False  # also reduce the norm in every l at data times
     gauge: str = "length"  # "length" or "velocity"
+    method: str = "sio"  # "sio" (SplitInteractionOperator) or "adi" (AlternatingDirectionImplicit)
 
     @property
     def npad(self):
@@ -275,7 +276,7 @@ def compile_sh(*, r_bound, r_points, l_bound, potential, field: Callable, mask, 
                g0: np.ndarray, basis: Sequence[Tuple[np.ndarray, int]] = (), store_every=1,
                test_charge=u.electron_charge, hydrogen_correction=True, rows=None,
                norm_by_l=False, layout: Optional[Tuple[int, int]] = None, gauge: str = "length",
-               test_mass=u.electron_mass_reduced) -> CompiledSH:
+               test_mass=u.electron_mass_reduced, method: str = "sio") -> CompiledSH:
     """
     Compile a SphericalHarmonicMesh + SplitInteractionOperator simulation.
     ``times`` is the reference time grid (sims.py:198-205); step n uses
@@ -283,11 +284,16 @@ def compile_sh(*, r_bound, r_points, l_bound, potential, field: Callable, mask, 
     t_{n+1} + time_step / 2 (mesh_operators.py:1011-1013). Velocity gauge
     (``gauge="velocity"``, SphericalHarmonicVelocityGaugeOperators): the
     vector potential of ``vector_potential_table`` and ``test_mass`` in the
-    interaction (mesh_operators.py:1143-1186). ``layout`` = (P, m) forces the
+    interaction (mesh_operators.py:1143-1186). ``method="adi"``:
+    AlternatingDirectionImplicit with the total Hamiltonian [H0 along r, E H_int
+    along l] (evolution_methods.py:46-77, mesh_operators.py:1020-1035), length
+    gauge, the field at t_{n+1} + time_step / 2. ``layout`` = (P, m) forces the
     device layout (the autotuner measures candidates this way).
     """
     if gauge not in ("length", "velocity"):
         raise DeviceError(f"unknown gauge {gauge!r}")
+    if method not in ("sio", "adi") or (method == "adi" and gauge != "length"):
+        raise DeviceError(f"unsupported SH method {method!r} with the {gauge} gauge")
     if gauge == "velocity" and (r_points < 3 or l_bound < 2):
         raise DeviceError("the velocity gauge needs r_points >= 3 and l_bound >= 2")
     if layout is not None:
@@ -326,7 +332,7 @@ def compile_sh(*, r_bound, r_points, l_bound, potential, field: Callable, mask, 
         fs=np.ascontiguousarray(fs), ts=np.ascontiguousarray(ts),
         g0=np.ascontiguousarray(to_positions(g0, P, m)),
         basis=np.ascontiguousarray(to_positions(bmat, P, m)), basis_l=bl, norm_by_l=bool(norm_by_l),
-        gauge=gauge,
+        gauge=gauge, method=method,
     )
 
 
@@ -671,7 +677,7 @@ class DeviceContext:
     def add_sh(self, c: CompiledSH) -> int:
         desc = _lib.ShDesc(c.r_points, c.l_bound, c.P, c.m, c.n_steps, c.store_every, len(c.basis_l), c.lb,
                            c.delta_r, c.corr0, c.centrifugal, 1 if c.norm_by_l else 0,
-                           1 if c.gauge == "velocity" else 0)
+                           1 if c.gauge == "velocity" else 0, 1 if c.method == "adi" else 0, 0)
         slot = ctypes.c_int32()
         basis = np.ascontiguousarray(c.basis, dtype=np.float64) if len(c.basis_l) else None
         bl = np.ascontiguousarray(c.basis_l, dtype=np.int32)

@@ -71,24 +71,28 @@ class SplitInteractionOperator(EvolutionMethod):
         gauge = sim.spec.operators.gauge
         if gauge not in (core.Gauge.LENGTH, core.Gauge.VELOCITY):
             raise NotImplementedError(f"unknown gauge {gauge}")
-        basis = []
-        for state in sim.spec.test_states:
-            comps = list(state)
-            if len(comps) != 1:
-                raise NotImplementedError("test states must have one spherical-harmonic component")
-            s = comps[0]
-            basis.append((np.real(mesh.get_radial_g_for_state(s)), s.l))
-        step = sim.spec.time_step
-        return engine.compile_sh(
-            r_bound=sim.spec.r_bound, r_points=sim.spec.r_points, l_bound=sim.spec.l_bound,
-            potential=_potential(sim), field=_field(sim), mask=_mask(sim), times=sim.times, time_step=step,
-            g0=mesh.g if g0 is None else g0, basis=basis, store_every=sim.spec.store_data_every,
-            test_charge=sim.spec.test_charge,
-            hydrogen_correction=sim.spec.operators.hydrogen_zero_angular_momentum_correction,
-            rows=getattr(sim.spec, "device_rows", None),
-            norm_by_l=data.NormBySphericalHarmonic in getattr(sim, "datastores_by_type", {}),
-            gauge="velocity" if gauge == core.Gauge.VELOCITY else "length", test_mass=sim.spec.test_mass,
-        )
+        return _compile_sh(sim, g0, gauge="velocity" if gauge == core.Gauge.VELOCITY else "length", method="sio")
+
+
+def _compile_sh(sim, g0, *, gauge, method):
+    mesh = sim.mesh
+    basis = []
+    for state in sim.spec.test_states:
+        comps = list(state)
+        if len(comps) != 1:
+            raise NotImplementedError("test states must have one spherical-harmonic component")
+        s = comps[0]
+        basis.append((np.real(mesh.get_radial_g_for_state(s)), s.l))
+    return engine.compile_sh(
+        r_bound=sim.spec.r_bound, r_points=sim.spec.r_points, l_bound=sim.spec.l_bound,
+        potential=_potential(sim), field=_field(sim), mask=_mask(sim), times=sim.times,
+        time_step=sim.spec.time_step, g0=mesh.g if g0 is None else g0, basis=basis,
+        store_every=sim.spec.store_data_every, test_charge=sim.spec.test_charge,
+        hydrogen_correction=sim.spec.operators.hydrogen_zero_angular_momentum_correction,
+        rows=getattr(sim.spec, "device_rows", None),
+        norm_by_l=data.NormBySphericalHarmonic in getattr(sim, "datastores_by_type", {}),
+        gauge=gauge, test_mass=sim.spec.test_mass, method=method,
+    )
 
 
 class AlternatingDirectionImplicit(EvolutionMethod):
@@ -96,11 +100,17 @@ class AlternatingDirectionImplicit(EvolutionMethod):
     Crank-Nicolson / ADI step (evolution_methods.py:46-77). On a LineMesh the
     total Hamiltonian is one operator, so a step is (1 + i tau H)^-1 (1 - i tau H)
     followed by the mask; on a CylindricalSliceMesh it is explicit rho, implicit
-    z, explicit z, implicit rho, then the mask.
+    z, explicit z, implicit rho, then the mask; on a SphericalHarmonicMesh
+    explicit H0 along r, implicit then explicit E H_int along l, implicit H0,
+    then the mask (mesh_operators.py:1020-1035).
     """
 
     def compile(self, sim, g0=None):
         mesh = sim.mesh
+        if isinstance(mesh, meshes.SphericalHarmonicMesh):
+            if sim.spec.operators.gauge != core.Gauge.LENGTH:
+                raise NotImplementedError("the device SphericalHarmonicMesh ADI step implements the length gauge")
+            return _compile_sh(sim, g0, gauge="length", method="adi")
         if isinstance(mesh, meshes.LineMesh):
             if sim.spec.operators.gauge != core.Gauge.LENGTH or isinstance(
                     sim.spec.operators, mesh_operators.LineVelocityGaugeOperators):

@@ -279,6 +279,25 @@ class SHOracle:
         split = self.split_operators(efield, tau)
         return (*split, *cn, *reversed(split))
 
+    def evolve_adi(self, g, time_step, time_now, nominal_time_step):
+        """AlternatingDirectionImplicit on the SH mesh (evolution_methods.py:46-77): over
+        chain([H0 (r), E H_int (l)], reversed) explicit, implicit, explicit, implicit
+        (mesh_operators.py:1020-1035), field at sim.time + spec.time_step / 2, then the mask."""
+        tau = time_step / (2 * u.hbar)
+        hint = self.field(time_now + nominal_time_step / 2) * self.l_interaction
+        ham = [(self.h0, "r"), (hint, "l")]
+        ops, explicit = [], True
+        for matrix, direction in itertools.chain(ham, reversed(ham)):
+            if explicit:
+                ops.append(_Op("dot", add_to_diagonal(-1j * tau * matrix, value=1), direction))
+            else:
+                ops.append(_Op("tdma", add_to_diagonal(1j * tau * matrix, value=1), direction))
+            explicit = not explicit
+        g = apply_sequentially(self.flatten, self.wrap, g, ops)
+        if self.mask_mesh is not None:
+            g = g * self.mask_mesh
+        return g
+
     def evolve(self, g, time_step, time_now, nominal_time_step):
         """QuantumMesh.evolve: field at sim.time + spec.time_step / 2 (mesh_operators.py:1011-1013)."""
         efield = self.field(time_now + nominal_time_step / 2)
@@ -299,9 +318,10 @@ class SHOracle:
         """SphericalHarmonicMesh.norm_by_l (meshes.py:1130-1133): |sum_r conj(g) g| delta_r per l."""
         return np.abs(np.sum(np.conj(g) * g, axis=1) * self.delta_r)
 
-    def run(self, g0, times, nominal_time_step, store_every=1, basis=(), by_l=False):
+    def run(self, g0, times, nominal_time_step, store_every=1, basis=(), by_l=False, adi=False):
         """MeshSimulation.run (sims.py:289-321): data first, then advance time_index and evolve.
-        by_l also returns the NormBySphericalHarmonic data (data.py:447-450), (n_data, L)."""
+        by_l also returns the NormBySphericalHarmonic data (data.py:447-450), (n_data, L);
+        adi evolves with AlternatingDirectionImplicit instead of SplitInteractionOperator."""
         g = np.array(g0, dtype=np.complex128)
         didx = set(data_indices(len(times), store_every).tolist())
         norms, ips, lnorm, nbl = [], [], [], []
@@ -314,7 +334,8 @@ class SHOracle:
                     nbl.append(self.norm_by_l(g))
             if i == len(times) - 1:
                 break
-            g = self.evolve(g, times[i + 1] - times[i], times[i + 1], nominal_time_step)
+            step = self.evolve_adi if adi else self.evolve
+            g = step(g, times[i + 1] - times[i], times[i + 1], nominal_time_step)
         out = (g, np.array(norms), np.array(ips, dtype=np.complex128), np.array(lnorm))
         return out + (np.array(nbl),) if by_l else out
 

@@ -14,7 +14,7 @@ from oracle import mesh_cpu, physics
 
 def sh_case(*, r_bound_bohr=30.0, r_points=200, l_bound=20, pulse_width_as=100.0, fluence_jcm2=1.0,
             phase=0.0, t_half_as=300.0, dt_as=1.0, n_basis_l=3, store_every=1, mask=True, rows=None,
-            norm_by_l=False, gauge="length", layout=None):
+            norm_by_l=False, gauge="length", layout=None, method="sio"):
     """SH + split operator (length or velocity gauge) with a sinc pulse, an absorber and a numeric 1s start."""
     r_bound = r_bound_bohr * u.bohr_radius
     pw = pulse_width_as * u.asec
@@ -45,8 +45,10 @@ def sh_case(*, r_bound_bohr=30.0, r_points=200, l_bound=20, pulse_width_as=100.0
     times = mesh_cpu.time_grid(-t_half_as * u.asec, t_half_as * u.asec, dt)
     comp = engine.compile_sh(r_bound=r_bound, r_points=r_points, l_bound=l_bound, potential=physics.coulomb,
                              field=field, mask=maskf, times=times, time_step=dt, g0=g0, basis=basis,
-                             store_every=store_every, rows=rows, norm_by_l=norm_by_l, gauge=gauge, layout=layout)
-    return dict(oracle=orc, compiled=comp, g0=g0, times=times, dt=dt, basis=basis, store_every=store_every)
+                             store_every=store_every, rows=rows, norm_by_l=norm_by_l, gauge=gauge, layout=layout,
+                             method=method)
+    return dict(oracle=orc, compiled=comp, g0=g0, times=times, dt=dt, basis=basis, store_every=store_every,
+                adi=method == "adi")
 
 
 def line_case(*, z_bound_bohr=50.0, z_points=500, n_steps=300, e0_au=0.05, omega_au=0.057, dt_au=0.0413,

@@ -1,0 +1,110 @@
+"""
+SphericalHarmonicMesh with AlternatingDirectionImplicit on the device
+(evolution_methods.py:46-77 with the total Hamiltonian of
+mesh_operators.py:1020-1035: explicit H0 along r, implicit and explicit E H_int
+along l, implicit H0, then the mask) against the CPU oracle: final wavefunction
+relative L2 <= 1e-10; norm, largest-l norm, overlaps and norm by l <= 1e-9
+absolute.
+"""
+
+import numpy as np
+import pytest
+
+from ionization_amd import mesh, potentials
+from ionization_amd import units as u
+from oracle import mesh_cpu, physics
+from tests.cases import sh_case
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+WF_RTOL = 1e-10
+OBS_ATOL = 1e-9
+
+
+def _run(ctx, comp, use_graph=True):
+    slot = ctx.add_sh(comp)
+    ctx.run(comp.n_steps + 1, use_graph=use_graph)
+    assert ctx.finished(0, slot)
+    res = ctx.download_sh(slot)
+    ctx.remove(0, slot)
+    return res
+
+
+def _check(res, case, by_l=False):
+    out = case["oracle"].run(case["g0"], case["times"], case["dt"], case["store_every"], basis=case["basis"],
+                             by_l=by_l, adi=True)
+    g, norms, ips, lnorm = out[:4]
+    assert rel_l2(res.g, g) <= WF_RTOL
+    assert np.max(np.abs(res.norms - norms)) <= OBS_ATOL
+    assert np.max(np.abs(res.lnorms - lnorm)) <= OBS_ATOL
+    assert np.max(np.abs(res.ips - ips)) <= OBS_ATOL
+    if by_l:
+        assert np.max(np.abs(res.norm_by_l - out[4])) <= OBS_ATOL
+
+
+@pytest.mark.parametrize(
+    "r_points, l_bound",
+    # odd / even sizes, a single l, a 3-point line, and l_bound > 64
+    [(200, 20), (201, 21), (7, 5), (3, 2), (40, 1), (150, 7), (96, 150)],
+)
+def test_sh_adi_matches_oracle(device_ctx, r_points, l_bound):
+    case = sh_case(r_points=r_points, l_bound=l_bound, t_half_as=100.0, method="adi")
+    comp = case["compiled"]
+    print(f"SH ADI r_points={r_points} l_bound={l_bound} P={comp.P} m={comp.m} steps={comp.n_steps}", flush=True)
+    _check(_run(device_ctx, comp), case)
+
+
+@pytest.mark.parametrize("rows", [2, 4, 6, 8, 10, 12, 14, 16])
+def test_sh_adi_every_rows_per_thread(device_ctx, rows):
+    """Every r-sweep instance (explicit and implicit) on a 700-point line."""
+    case = sh_case(r_points=700, l_bound=6, t_half_as=20.0, rows=rows, method="adi")
+    comp = case["compiled"]
+    print(f"SH ADI rows={rows} P={comp.P}", flush=True)
+    assert comp.m == rows
+    _check(_run(device_ctx, comp), case)
+
+
+def test_sh_adi_norm_by_l_sparse_data_and_direct_launches(device_ctx):
+    case = sh_case(r_points=120, l_bound=10, t_half_as=60.0, store_every=7, norm_by_l=True, method="adi")
+    graph = _run(device_ctx, case["compiled"])
+    _check(graph, case, by_l=True)
+    direct = _run(device_ctx, case["compiled"], use_graph=False)
+    assert np.array_equal(direct.g, graph.g)
+
+
+def test_sh_methods_in_one_batch(device_ctx):
+    """ADI, split operator and velocity gauge in one context: each bit-identical to its run alone."""
+    cases = [sh_case(r_points=200, l_bound=20, t_half_as=50.0, method="adi"),
+             sh_case(r_points=200, l_bound=20, t_half_as=50.0),
+             sh_case(r_points=333, l_bound=15, t_half_as=30.0, fluence_jcm2=3.0, method="adi"),
+             sh_case(r_points=150, l_bound=9, t_half_as=40.0, gauge="velocity")]
+    alone = [_run(device_ctx, c["compiled"]) for c in cases]
+    slots = [device_ctx.add_sh(c["compiled"]) for c in cases]
+    device_ctx.run(max(c["compiled"].n_steps for c in cases) + 1)
+    for c, a, s in zip(cases, alone, slots):
+        m = device_ctx.download_sh(s)
+        device_ctx.remove(0, s)
+        assert np.array_equal(m.g, a.g) and np.array_equal(m.norms, a.norms)
+
+
+def test_sh_adi_api_matches_oracle():
+    """The drop-in path (SphericalHarmonicSpecification + AlternatingDirectionImplicit: the
+    reference benchmark's SphericalHarmonic_LAG_CN_LEN) with a pulse, against the oracle."""
+    pw, fl, rb = 80 * u.asec, 1.0 * u.Jcm2, 30 * u.bohr_radius
+    spec = mesh.SphericalHarmonicSpecification(
+        "adi", evolution_method=mesh.AlternatingDirectionImplicit(),
+        electric_potential=potentials.SincPulse(pulse_width=pw, fluence=fl, phase=0.3),
+        mask=potentials.RadialCosineMask(25 * u.bohr_radius, rb),
+        time_initial=-60 * u.asec, time_final=60 * u.asec, time_step=1 * u.asec,
+        r_bound=rb, r_points=200, l_bound=16)
+    sim = spec.to_sim()
+    g0 = sim.mesh.g.copy()
+    sim.run()
+    orc = mesh_cpu.SHOracle(
+        r_bound=rb, r_points=spec.r_points, l_bound=spec.l_bound, potential=physics.coulomb,
+        field=lambda t: physics.sinc_pulse_field(t, pulse_width=pw, fluence=fl, phase=0.3),
+        mask=lambda r: physics.radial_cosine_mask(r, 25 * u.bohr_radius, rb))
+    g, norms, _, _ = orc.run(g0, sim.times, spec.time_step, spec.store_data_every, adi=True)
+    assert rel_l2(sim.mesh.g, g) <= WF_RTOL
+    assert np.max(np.abs(sim.data.norm - norms)) <= OBS_ATOL

@@ -569,3 +569,66 @@ def test_line_split_operator_is_unitary_without_mask(gauge):
     g = rng.standard_normal(40) + 1j * rng.standard_normal(40)
     gf, norms, _ = o.run(g, np.arange(30) * 2 * u.asec)
     assert abs(norms[-1] - norms[0]) < 1e-12 * norms[0]
+
+
+# ------------------------------------------------------------------ SphericalHarmonicMesh ADI
+
+
+@pytest.mark.parametrize("r_points, l_bound", [(6, 4), (5, 3)])
+def test_sh_adi_step_matches_dense_composition(r_points, l_bound):
+    """AlternatingDirectionImplicit on the SH mesh: (1 + i tau H0)^-1 (1 - i tau E H_int)
+    (1 + i tau E H_int)^-1 (1 - i tau H0), then the mask, with H0 along r and H_int along l
+    (evolution_methods.py:46-77, mesh_operators.py:1020-1035), built densely with explicit
+    permutation matrices between the flattenings."""
+    rb = 8 * u.bohr_radius
+    maskf = lambda r: physics.radial_cosine_mask(r, 0.5 * rb, rb)
+    fld = lambda t: 0.3 * u.atomic_electric_field * np.cos(t / (20 * u.asec))
+    o = _tiny_sh(r_points, l_bound, field=fld, mask=maskf)
+    rng = np.random.default_rng(1)
+    g = rng.standard_normal((l_bound, r_points)) + 1j * rng.standard_normal((l_bound, r_points))
+    dt, t1 = 2 * u.asec, 7 * u.asec
+    out = o.evolve_adi(g, dt, t1, dt)
+    tau = dt / (2 * u.hbar)
+    N = r_points * l_bound
+    H0 = _dense_h0_closed_form(o)
+    hint = np.zeros((N, N))  # l-flattening: (k, k+1) -> E (-q) r_{k//L} c_l(k % L)
+    for k in range(N - 1):
+        l, j = k % l_bound, k // l_bound
+        if (k + 1) % l_bound != 0:
+            cl = (l + 1) / np.sqrt((2 * l + 1) * (2 * l + 3))
+            hint[k, k + 1] = hint[k + 1, k] = fld(t1 + dt / 2) * (-u.electron_charge) * o.r[j] * cl
+    Q = _perm_R_to_L(o)
+    eye = np.eye(N)
+    vR = (eye - 1j * tau * H0) @ g.reshape(-1)
+    vL = Q @ vR
+    vL = (eye - 1j * tau * hint) @ np.linalg.solve(eye + 1j * tau * hint, vL)
+    vR = np.linalg.solve(eye + 1j * tau * H0, Q.T @ vL)
+    ref = vR.reshape(l_bound, r_points) * maskf(o.r_mesh)
+    assert rel_l2(out, ref) < 1e-12
+
+
+def test_sh_adi_field_free_stationary_and_its_invariant():
+    o = mesh_cpu.SHOracle(r_bound=30 * u.bohr_radius, r_points=120, l_bound=6, potential=physics.coulomb,
+                          field=lambda t: 0.0 * t)
+    n = o.r_points
+    blk = o.h0.tocsr()[:n, :n].toarray().real
+    _, v = np.linalg.eigh(blk)
+    phi = v[:, 0] / np.sqrt(o.delta_r * np.sum(v[:, 0] ** 2))
+    g = np.zeros((6, n), dtype=np.complex128)
+    g[0] = phi
+    times = mesh_cpu.time_grid(0, 50 * u.asec, 1 * u.asec)
+    _, norms, ips, _ = o.run(g, times, 1 * u.asec, basis=[(phi, 0)], adi=True)
+    assert np.allclose(norms[0], norms[-1], atol=1e-14)
+    assert np.allclose(np.abs(ips[0]) ** 2, np.abs(ips[-1]) ** 2, atol=1e-14)
+    # with a field the step is S^-1 (C_int C_0) S, S = 1 + i tau H0 (C_0 = (1 - i tau H0) S^-1): the
+    # norm of S g is conserved exactly (unit-modulus Cayley factors), not the norm of g
+    fld = lambda t: physics.sinc_pulse_field(t, pulse_width=50 * u.asec, fluence=1 * u.Jcm2)
+    o2 = mesh_cpu.SHOracle(r_bound=20 * u.bohr_radius, r_points=60, l_bound=8, potential=physics.coulomb, field=fld)
+    rng = np.random.default_rng(5)
+    g2 = rng.standard_normal((8, 60)) + 1j * rng.standard_normal((8, 60))
+    times = mesh_cpu.time_grid(-40 * u.asec, 40 * u.asec, 1 * u.asec)
+    gN, norms2, _, _ = o2.run(g2, times, 1 * u.asec, adi=True)
+    S = mesh_cpu.add_to_diagonal((1j * (1 * u.asec) / (2 * u.hbar)) * o2.h0, value=1)
+    s0, sN = np.linalg.norm(S.dot(g2.reshape(-1))), np.linalg.norm(S.dot(gN.reshape(-1)))
+    assert abs(sN - s0) < 1e-12 * s0
+    assert abs(norms2[-1] - norms2[0]) > 1e-6 * norms2[0]  # while the plain norm is not conserved
