@@ -121,6 +121,31 @@ def time_velocity(params, store_every, steps, warmup):
     return out
 
 
+def time_sh_adi(params, store_every, steps, warmup):
+    """
+    The same scan draws with AlternatingDirectionImplicit (the reference benchmark's
+    SphericalHarmonic_LAG_CN_LEN): per step the L-phase (mask, observables), explicit H0
+    along r, the l-direction Cayley factor (a Thomas sweep per position that keeps one real
+    pivot factor per row in the idle state buffer), implicit H0 along r.
+    """
+    from ionization_amd import mesh, workloads
+
+    t = time.perf_counter()
+    compiled = []
+    for p in params:
+        sim = workloads.sh_scan_spec(p, store_data_every=store_every,
+                                     evolution_method=mesh.AlternatingDirectionImplicit()).to_sim()
+        compiled.append(sim.spec.evolution_method.compile(sim))
+    build_s = time.perf_counter() - t
+    b = time_batched(compiled, steps, warmup)
+    rate = b["points"] * steps / b["wall_s"]
+    out = {"workload": f"sh_scan draws ({len(compiled)} simulations), AlternatingDirectionImplicit, one hipGraph "
+                       "replay per step", "value": rate, "unit": "mesh-point-steps/s",
+           "ms_per_step": b["wall_s"] * 1e3 / steps, "build_s": build_s}
+    log(f"SH ADI: {out['ms_per_step']:.3f} ms/step, {rate / 1e9:.2f} G mesh-point-steps/s")
+    return out
+
+
 def time_plain(steps, warmup):
     from ionization_amd import workloads
     from ionization_amd.engine import DeviceContext
@@ -556,6 +581,7 @@ def main():
                     help="also measure every candidate SH layout per radial size class and time the scan "
                          "under the measured layouts (reported as 'autotuned')")
     ap.add_argument("--velocity", type=int, default=1, help="0 skips the velocity-gauge scan")
+    ap.add_argument("--sh-adi", type=int, default=1, help="0 skips the SphericalHarmonicMesh ADI scan")
     ap.add_argument("--cyl-sims", type=int, default=32, help="0 skips the cylindrical ADI workload")
     ap.add_argument("--cyl-steps", type=int, default=100)
     ap.add_argument("--line-batch", type=int, default=256, help="0 skips the LineMesh workload")
@@ -638,6 +664,7 @@ def main():
     step_gbs = rate * BYTES_PER_POINT_STEP / 1e9
 
     vel = time_velocity(params, args.store_every, args.steps, args.warmup) if args.velocity else None
+    sh_adi = time_sh_adi(params, args.store_every, args.steps, args.warmup) if args.sh_adi else None
     cyl = time_cyl(args.cyl_sims, args.seed, args.cyl_steps, 10) if args.cyl_sims > 0 else None
     line = time_line(args.line_batch, min(args.cpu_seconds, 5.0)) if args.line_batch > 0 else None
     served = (time_serve(args.serve_requests, args.serve_rate, args.seed, args.serve_pool_gb)
@@ -692,6 +719,7 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baseline_parallel": cpu_par,
         "velocity_gauge": vel,
+        "sh_adi": sh_adi,
         "cyl_adi": cyl,
         "line_cn": line,
         "served_stream": served,
