@@ -15,6 +15,7 @@ Cases (parameters are stored in the file; the builders are tests/cases.py):
   shv   SphericalHarmonicMesh velocity gauge, 61 r x 9 l, 60 steps, data every 5
   lsl   LineMesh split operator, length gauge, 301 z, 60 steps, strong unramped field
   lsv   LineMesh split operator, velocity gauge, 301 z, 60 steps, strong unramped field
+  sha   SphericalHarmonicMesh AlternatingDirectionImplicit, 61 r x 9 l, 60 steps, data every 5
 """
 
 import json
@@ -37,6 +38,7 @@ PARAMS = {
     "shv": dict(r_points=61, l_bound=9, t_half_as=30.0, store_every=5),
     "lsl": dict(z_points=301, n_steps=60, store_every=10, e0_au=2.0, omega_au=0.5),
     "lsv": dict(z_points=301, n_steps=60, store_every=10, e0_au=2.0, omega_au=0.5),
+    "sha": dict(r_points=61, l_bound=9, t_half_as=30.0, store_every=5),
 }
 
 
@@ -85,6 +87,13 @@ def build():
                          gauge=gauge, e0_au=p["e0_au"], omega_au=p["omega_au"], ramp=False)
         g, norms, ips = case["oracle"].run(case["g0"], case["times"], p["store_every"], basis=case["basis"])
         out.update({f"{key}_g0": case["g0"], f"{key}_g": g, f"{key}_norms": norms, f"{key}_ips": ips})
+
+    p = PARAMS["sha"]
+    case = sh_case(r_points=p["r_points"], l_bound=p["l_bound"], t_half_as=p["t_half_as"], store_every=p["store_every"],
+                   method="adi")
+    g, norms, ips, lnorm = case["oracle"].run(case["g0"], case["times"], case["dt"], p["store_every"],
+                                             basis=case["basis"], adi=True)
+    out.update(sha_g0=case["g0"], sha_g=g, sha_norms=norms, sha_ips=ips, sha_lnorms=lnorm)
     return out
 
 

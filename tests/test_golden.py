@@ -90,3 +90,15 @@ def test_line_split_operator_fixture(gold, key, gauge):
     g, norms, ips = case["oracle"].run(case["g0"], case["times"], p["store_every"], basis=case["basis"])
     assert rel_l2(g, gold[f"{key}_g"]) <= 1e-13
     assert np.max(np.abs(norms - gold[f"{key}_norms"])) <= 1e-13
+
+
+def test_sh_adi_fixture(gold):
+    p = gold["params"]["sha"]
+    case = sh_case(r_points=p["r_points"], l_bound=p["l_bound"], t_half_as=p["t_half_as"], store_every=p["store_every"],
+                   method="adi")
+    assert np.array_equal(case["g0"], gold["sha_g0"])
+    g, norms, ips, lnorm = case["oracle"].run(case["g0"], case["times"], case["dt"], p["store_every"],
+                                             basis=case["basis"], adi=True)
+    assert rel_l2(g, gold["sha_g"]) <= 1e-13
+    assert np.max(np.abs(norms - gold["sha_norms"])) <= 1e-13
+    assert np.max(np.abs(ips - gold["sha_ips"])) <= 1e-13

@@ -99,3 +99,17 @@ def test_line_split_operator_matches_golden(device_ctx, gold, key, gauge):
     assert rel_l2(g, gold[f"{key}_g"]) <= WF_RTOL
     assert np.max(np.abs(norms - gold[f"{key}_norms"])) <= OBS_ATOL
     assert np.max(np.abs(ips - gold[f"{key}_ips"])) <= OBS_ATOL
+
+
+def test_sh_adi_matches_golden(device_ctx, gold):
+    p = gold["params"]["sha"]
+    c = sh_case(r_points=p["r_points"], l_bound=p["l_bound"], t_half_as=p["t_half_as"],
+                store_every=p["store_every"], method="adi")["compiled"]
+    s = device_ctx.add_sh(c)
+    device_ctx.run(c.n_steps + 1)
+    r = device_ctx.download_sh(s)
+    device_ctx.remove(0, s)
+    assert rel_l2(r.g, gold["sha_g"]) <= WF_RTOL
+    assert np.max(np.abs(r.norms - gold["sha_norms"])) <= OBS_ATOL
+    assert np.max(np.abs(r.lnorms - gold["sha_lnorms"])) <= OBS_ATOL
+    assert np.max(np.abs(r.ips - gold["sha_ips"])) <= OBS_ATOL
