@@ -55,9 +55,9 @@ def test_served_results_equal_alone_results():
         assert np.array_equal(r.ips, ips), r.kind
 
 
-def test_served_velocity_gauge_and_line_split_operator_equal_alone():
-    """The velocity-gauge SH path (its own L-phase instance and h2 chain) and the Line split
-    operator, served beside length-gauge simulations, give their alone results bit for bit."""
+def test_served_velocity_gauge_adi_and_line_split_operator_equal_alone():
+    """The velocity-gauge SH path (its own L-phase instance and h2 chain), SH ADI and the Line
+    split operator, served beside length-gauge simulations, give their alone results bit for bit."""
     comps = [
         sh_case(r_points=200, l_bound=20, t_half_as=30.0, gauge="velocity")["compiled"],
         sh_case(r_points=150, l_bound=11, t_half_as=20.0)["compiled"],
@@ -66,8 +66,9 @@ def test_served_velocity_gauge_and_line_split_operator_equal_alone():
         sh_case(r_points=333, l_bound=15, t_half_as=12.0, fluence_jcm2=3.0, gauge="velocity")["compiled"],
         line_case(z_points=257, n_steps=33, method="sio", gauge="length", e0_au=2.0, omega_au=0.5,
                   ramp=False)["compiled"],
+        sh_case(r_points=180, l_bound=13, t_half_as=25.0, method="adi")["compiled"],
     ]
-    arrivals = [0.0, 0.0, 0.01, 0.03, 0.05]
+    arrivals = [0.0, 0.0, 0.01, 0.03, 0.05, 0.06]
     reqs = [serve.Request(i, type(c).__name__, a, (lambda c=c: c)) for i, (c, a) in enumerate(zip(comps, arrivals))]
     pool = max(sum(serve.Server._reserve(c) for c in comps) * 2 // 3, max(serve.Server._reserve(c) for c in comps))
     with serve.Server(pool_bytes=pool, chunk_steps=9) as srv:
