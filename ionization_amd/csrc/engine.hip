@@ -188,6 +188,7 @@ struct tdse_ctx {
     std::vector<int> sh_order;  // sh_buckets by decreasing work
     std::vector<Bucket> sh_adi_buckets;  // AlternatingDirectionImplicit l-lines (slot, l) by (rows, chunks)
     DevVec<int2> adi_waves;              // AlternatingDirectionImplicit positions: (slot, first position of a wave)
+    std::vector<Bucket> sh_adi_lbuckets;  // ADI position groups (slot, group) of the partition l-sweep, by rows
     hipStream_t side[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
     std::vector<Bucket> line_buckets;
@@ -259,7 +260,8 @@ int rebuild_tables(tdse_ctx* ctx) {
     std::map<std::pair<int, int>, std::vector<int2>> vpb[2];    // velocity l-pairs by parity, (rows, chunks)
     std::map<std::pair<int, int>, std::vector<int2>> fpb;       // velocity odd l-pairs solved in one workgroup
     std::map<std::pair<int, int>, std::vector<int2>> adb;       // ADI l-lines, (rows, chunks)
-    std::vector<int2> adw;                                      // ADI position waves
+    std::vector<int2> adw;                                      // ADI position waves (Thomas l-sweep)
+    std::map<std::pair<int, int>, std::vector<int2>> alb;       // ADI position groups (partition l-sweep), (rows, 0)
     ctx->sh_norm_by_l = false;
     for (int pass = 0; pass < 2; ++pass)
     for (int s = 0; s < (int)ctx->sh.size(); ++s) {
@@ -283,7 +285,13 @@ int rebuild_tables(tdse_ctx* ctx) {
         for (int b = 0; b < h.dev.n_basis; ++b) jobs.push_back(make_int2(s, b));
         if (h.adi) {
             for (int l = 0; l < h.dev.L; ++l) adb[{h.dev.m, h.dev.P}].push_back(make_int2(s, l));
-            for (int w = 0; w < wpl; ++w) adw.push_back(make_int2(s, w * TDSE_WAVE));
+            // l-direction: the partition solver where kAdiChunks chunks of at most 16 rows cover l_bound
+            const int ml = 2 * ((h.dev.L + 2 * kAdiChunks - 1) / (2 * kAdiChunks));
+            if (ml <= 16) {
+                for (int gq = 0; gq < h.dev.npad / kAdiLanes; ++gq) alb[{std::max(ml, 2), 0}].push_back(make_int2(s, gq));
+            } else {
+                for (int w = 0; w < wpl; ++w) adw.push_back(make_int2(s, w * TDSE_WAVE));
+            }
             continue;
         }
         auto& v = shb[{h.dev.m, h.dev.P}];
@@ -342,6 +350,7 @@ int rebuild_tables(tdse_ctx* ctx) {
     for (auto& kv : fpb) shb[kv.first];  // a bucket for every layout with fused pairs, even without single lines
     if ((rc = fill(ctx->sh_buckets, shb))) return rc;
     if ((rc = fill(ctx->sh_adi_buckets, adb))) return rc;
+    if ((rc = fill(ctx->sh_adi_lbuckets, alb))) return rc;
     if ((rc = devvec_upload(ctx, ctx->adi_waves, adw))) return rc;
     for (auto& b : ctx->sh_buckets) {
         for (int par = 0; par < 2; ++par) {
@@ -459,6 +468,26 @@ void dispatch_adi_r(tdse_ctx* ctx, const Bucket& b) {
         case 12: b.P > 256 ? launch_adi_r<12, 512, 1, EXPL>(ctx, b) : launch_adi_r<12, 256, 2, EXPL>(ctx, b); break;
         case 14: b.P > 256 ? launch_adi_r<14, 512, 1, EXPL>(ctx, b) : launch_adi_r<14, 256, 2, EXPL>(ctx, b); break;
         default: b.P > 256 ? launch_adi_r<16, 512, 1, EXPL>(ctx, b) : launch_adi_r<16, 256, 1, EXPL>(ctx, b); break;
+    }
+}
+
+template <int M>
+void launch_adi_lp(tdse_ctx* ctx, const Bucket& b) {
+    const size_t lds = 3 * (size_t)kAdiChunks * kAdiLanes * sizeof(double2);
+    hipLaunchKernelGGL(k_sh_adi_lp<M>, dim3((unsigned)b.lines.count), dim3(kAdiChunks * kAdiLanes), lds, ctx->stream,
+                       ctx->d_sh, b.lines.ptr, (int)b.lines.count, ctx->d_cl);
+}
+
+void dispatch_adi_lp(tdse_ctx* ctx, const Bucket& b) {
+    switch (b.M) {
+        case 2: launch_adi_lp<2>(ctx, b); break;
+        case 4: launch_adi_lp<4>(ctx, b); break;
+        case 6: launch_adi_lp<6>(ctx, b); break;
+        case 8: launch_adi_lp<8>(ctx, b); break;
+        case 10: launch_adi_lp<10>(ctx, b); break;
+        case 12: launch_adi_lp<12>(ctx, b); break;
+        case 14: launch_adi_lp<14>(ctx, b); break;
+        default: launch_adi_lp<16>(ctx, b); break;
     }
 }
 
@@ -652,9 +681,11 @@ int launch_step(tdse_ctx* ctx, bool cyl_pre) {
         // AlternatingDirectionImplicit simulations: explicit H0, the l-direction factor, implicit H0
         if (!ctx->sh_adi_buckets.empty()) {
             for (auto& b : ctx->sh_adi_buckets) dispatch_adi_r<true>(ctx, b);
+            for (auto& b : ctx->sh_adi_lbuckets) dispatch_adi_lp(ctx, b);
             const int na = (int)ctx->adi_waves.count;
-            hipLaunchKernelGGL(k_sh_adi_l, dim3((na * TDSE_WAVE + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh,
-                               ctx->adi_waves.ptr, na, ctx->d_cl);
+            if (na > 0)
+                hipLaunchKernelGGL(k_sh_adi_l, dim3((na * TDSE_WAVE + 255) / 256), dim3(256), 0, ctx->stream,
+                                   ctx->d_sh, ctx->adi_waves.ptr, na, ctx->d_cl);
             for (auto& b : ctx->sh_adi_buckets) dispatch_adi_r<false>(ctx, b);
         }
         const int ns = (int)ctx->sh.size();
@@ -802,8 +833,8 @@ int tdse_ctx_destroy(tdse_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     drop_graph(ctx);
-    for (auto* v : {&ctx->sh_buckets, &ctx->sh_adi_buckets, &ctx->line_buckets, &ctx->cyl_z_buckets,
-                    &ctx->cyl_r_buckets})
+    for (auto* v : {&ctx->sh_buckets, &ctx->sh_adi_buckets, &ctx->sh_adi_lbuckets, &ctx->line_buckets,
+                    &ctx->cyl_z_buckets, &ctx->cyl_r_buckets})
         for (auto& b : *v) free_bucket(b);
     if (ctx->adi_waves.ptr) (void)hipFree(ctx->adi_waves.ptr);
     if (ctx->cyl_jobs.ptr) (void)hipFree(ctx->cyl_jobs.ptr);

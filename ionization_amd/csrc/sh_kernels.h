@@ -859,6 +859,61 @@ __global__ void __launch_bounds__(256) k_sh_adi_l(ShSim* __restrict__ sims, cons
     }
 }
 
+// The same l-direction factor by the partition solver, for l_bound <= 32 M:
+// one workgroup of kAdiLanes x kAdiChunks threads per group of kAdiLanes
+// consecutive positions; thread t works on position lane t % kAdiLanes and
+// rows l = c M .. c M + M - 1 of chunk c = t / kAdiLanes (a wave reads full
+// 128 B runs of each row). The explicit right-hand side (1 - i A) x is formed
+// in the solver's first pass, so every element is read and written once.
+constexpr int kAdiLanes = 8, kAdiChunks = 32;
+
+template <int M>
+__global__ void __launch_bounds__(kAdiLanes * kAdiChunks) k_sh_adi_lp(ShSim* __restrict__ sims,
+                                                                      const int2* __restrict__ groups, int n_groups,
+                                                                      const double* __restrict__ cl) {
+    extern __shared__ double2 lds[];  // 3 * kAdiChunks * kAdiLanes
+    constexpr int LANES = kAdiLanes;
+    const int job = xcd_remap(blockIdx.x, gridDim.x);
+    if (job >= n_groups) return;
+    const int2 gt = groups[job];
+    ShSim& S = sims[gt.x];
+    const int n = S.step;
+    if (!S.active || n >= S.N) return;  // uniform across the block
+    const int t = threadIdx.x;
+    const int ln = t % LANES, c = t / LANES;
+    const int L = S.L, npad = S.npad;
+    const int p = gt.y * LANES + ln;
+    const double f = ld1(S.fa, n) * ld1(S.tab, SH_RQ * npad + p);  // zero at padding positions
+    double2* __restrict__ g = sh_state(S, n + 1) + p;
+    double2 x[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        const int l = c * M + i;
+        x[i] = (l < L) ? ld2s(g, (size_t)l * npad) : cmk(0.0, 0.0);
+    }
+    // rows in order: sub-diagonal a_{l-1}, zero diagonal, super-diagonal a_l (zero from l = L - 1 on)
+    auto coef = [&](int i, double& hm, double& h, double& hp) {
+        const int l = c * M + i;
+        hm = (l >= 1 && l < L) ? f * ld1(cl, l - 1) : 0.0;
+        h = 0.0;
+        hp = (l + 1 < L) ? f * ld1(cl, l) : 0.0;
+    };
+    auto coef2 = [&](int i, double& hm, double& h) {
+        const int l = c * M + i;
+        hm = (l >= 1 && l < L) ? f * ld1(cl, l - 1) : 0.0;
+        h = 0.0;
+    };
+    double2 xl, xr;
+    chunk_neighbours<M, LANES>(x, c, kAdiChunks, lds, xl, xr, false, ln);
+    cn_implicit_lowreg<M, decltype(coef), decltype(coef2), true, LANES>(x, c, kAdiChunks, coef, coef2, lds, xl, xr,
+                                                                        ln);
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        const int l = c * M + i;
+        if (l < L) st2s(g, (size_t)l * npad, x[i]);
+    }
+}
+
 // Advance every active simulation's step counter after a full graph step.
 __global__ void k_sh_advance(ShSim* sims, int n_sims) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;

@@ -45,8 +45,9 @@ def _check(res, case, by_l=False):
 
 @pytest.mark.parametrize(
     "r_points, l_bound",
-    # odd / even sizes, a single l, a 3-point line, and l_bound > 64
-    [(200, 20), (201, 21), (7, 5), (3, 2), (40, 1), (150, 7), (96, 150)],
+    # odd / even sizes, a single l, a 3-point line, l_bound > 64; the l-direction partition solve
+    # up to l_bound 512 (16 rows per thread) and the Thomas sweep beyond it
+    [(200, 20), (201, 21), (7, 5), (3, 2), (40, 1), (150, 7), (96, 150), (12, 512), (12, 513), (20, 600)],
 )
 def test_sh_adi_matches_oracle(device_ctx, r_points, l_bound):
     case = sh_case(r_points=r_points, l_bound=l_bound, t_half_as=100.0, method="adi")
