@@ -146,6 +146,58 @@ def time_sh_adi(params, store_every, steps, warmup):
     return out
 
 
+def time_reference_specs():
+    """
+    The specifications of the reference's own benchmark (benchmarks/benchmark_mesh_evolution.py:
+    43-166) that are on the device path, built and run one at a time through the drop-in API
+    and counted the way it counts: spacetime points = len(times) x mesh points (:22-23) over the
+    process CPU time of spec.to_sim() + sim.run() (:14-19, 36), with the wall time beside it.
+    """
+    from ionization_amd import mesh, potentials, states
+    from ionization_amd import units as u
+
+    energy_spacing, test_mass = 0.1 * u.eV, u.electron_mass
+    qho = potentials.HarmonicOscillator.from_energy_spacing_and_mass(energy_spacing=energy_spacing, mass=test_mass)
+    sine = potentials.SineWave.from_photon_energy(photon_energy=energy_spacing, amplitude=0.0001 * u.atomic_electric_field)
+    line_kw = dict(time_initial=0, time_final=1 * sine.period, time_step=0.001 * sine.period, internal_potential=qho,
+                   initial_state=states.QHOState.from_potential(qho, n=0, mass=test_mass), z_bound=100 * u.nm,
+                   z_points=2 ** 12, store_data_every=-1, test_mass=test_mass, electric_potential=sine)
+    three_d = dict(time_initial=0, time_final=1 * u.fsec, time_step=1 * u.asec, store_data_every=-1)
+    sh_kw = dict(three_d, r_bound=50 * u.bohr_radius, r_points=250, l_bound=100, use_numeric_eigenstates=False)
+    cyl_kw = dict(three_d, rho_bound=50 * u.bohr_radius, rho_points=100, z_bound=50 * u.bohr_radius, z_points=250)
+    specs = [
+        mesh.LineSpecification("Line_CN_LEN", operators=mesh.LineLengthGaugeOperators(),
+                               evolution_method=mesh.AlternatingDirectionImplicit(), **line_kw),
+        mesh.LineSpecification("Line_SO_LEN", operators=mesh.LineLengthGaugeOperators(),
+                               evolution_method=mesh.SplitInteractionOperator(), **line_kw),
+        mesh.LineSpecification("Line_SO_VEL", operators=mesh.LineVelocityGaugeOperators(),
+                               evolution_method=mesh.SplitInteractionOperator(), **line_kw),
+        mesh.CylindricalSliceSpecification("CylindricalSlice_CN_LEN", evolution_method=mesh.AlternatingDirectionImplicit(),
+                                           **cyl_kw),
+        mesh.SphericalHarmonicSpecification("SphericalHarmonic_LAG_CN_LEN",
+                                            operators=mesh.SphericalHarmonicLengthGaugeOperators(),
+                                            evolution_method=mesh.AlternatingDirectionImplicit(), **sh_kw),
+        mesh.SphericalHarmonicSpecification("SphericalHarmonic_LAG_SO_LEN",
+                                            operators=mesh.SphericalHarmonicLengthGaugeOperators(),
+                                            evolution_method=mesh.SplitInteractionOperator(), **sh_kw),
+        mesh.SphericalHarmonicSpecification("SphericalHarmonic_LAG_SO_VEL",
+                                            operators=mesh.SphericalHarmonicVelocityGaugeOperators(),
+                                            evolution_method=mesh.SplitInteractionOperator(), **sh_kw),
+    ]
+    out = {}
+    for spec in specs:
+        w0, c0 = time.perf_counter(), time.process_time()
+        sim = spec.to_sim()
+        sim.run()
+        wall, cpu = time.perf_counter() - w0, time.process_time() - c0
+        pts = len(sim.times) * int(np.prod(sim.mesh.g.shape))
+        out[spec.name] = {"spacetime_points": pts, "cpu_s": cpu, "wall_s": wall,
+                          "pts_per_cpu_s": pts / cpu, "pts_per_wall_s": pts / wall}
+        log(f"reference benchmark spec {spec.name}: {pts / wall / 1e6:.1f} M spacetime points per wall second "
+            f"({pts / cpu / 1e6:.1f} M per CPU second)")
+    return out
+
+
 def time_plain(steps, warmup):
     from ionization_amd import workloads
     from ionization_amd.engine import DeviceContext
@@ -582,6 +634,8 @@ def main():
                          "under the measured layouts (reported as 'autotuned')")
     ap.add_argument("--velocity", type=int, default=1, help="0 skips the velocity-gauge scan")
     ap.add_argument("--sh-adi", type=int, default=1, help="0 skips the SphericalHarmonicMesh ADI scan")
+    ap.add_argument("--reference-specs", type=int, default=1,
+                    help="0 skips the reference benchmark's own specifications (run one at a time)")
     ap.add_argument("--cyl-sims", type=int, default=32, help="0 skips the cylindrical ADI workload")
     ap.add_argument("--cyl-steps", type=int, default=100)
     ap.add_argument("--line-batch", type=int, default=256, help="0 skips the LineMesh workload")
@@ -665,6 +719,7 @@ def main():
 
     vel = time_velocity(params, args.store_every, args.steps, args.warmup) if args.velocity else None
     sh_adi = time_sh_adi(params, args.store_every, args.steps, args.warmup) if args.sh_adi else None
+    ref_specs = time_reference_specs() if args.reference_specs else None
     cyl = time_cyl(args.cyl_sims, args.seed, args.cyl_steps, 10) if args.cyl_sims > 0 else None
     line = time_line(args.line_batch, min(args.cpu_seconds, 5.0)) if args.line_batch > 0 else None
     served = (time_serve(args.serve_requests, args.serve_rate, args.seed, args.serve_pool_gb)
@@ -720,6 +775,7 @@ def main():
         "cpu_baseline_parallel": cpu_par,
         "velocity_gauge": vel,
         "sh_adi": sh_adi,
+        "reference_benchmark_specs": ref_specs,
         "cyl_adi": cyl,
         "line_cn": line,
         "served_stream": served,

@@ -50,6 +50,41 @@ class CoulombPotential(PotentialEnergy):
         return f"CoulombPotential(charge={self.charge / u.proton_charge} e)"
 
 
+class HarmonicOscillator(PotentialEnergy):
+    """1/2 k (r - center)^2, constant beyond an optional cutoff (``potentials/static.py:113-233``)."""
+
+    def __init__(self, spring_constant: float = 4.20521 * u.N / u.m, center: float = 0.0, cutoff_distance=None):
+        self.spring_constant = spring_constant
+        self.center = center
+        self.cutoff_distance = cutoff_distance
+
+    @classmethod
+    def from_frequency_and_mass(cls, omega: float = 1.5192675e15 * u.Hz, mass: float = u.electron_mass, **kwargs):
+        return cls(spring_constant=mass * (omega ** 2), **kwargs)
+
+    @classmethod
+    def from_ground_state_energy_and_mass(cls, ground_state_energy: float = 0.5 * u.eV, mass: float = u.electron_mass,
+                                          **kwargs):
+        return cls.from_frequency_and_mass(omega=2 * ground_state_energy / u.hbar, mass=mass, **kwargs)
+
+    @classmethod
+    def from_energy_spacing_and_mass(cls, energy_spacing: float = 1 * u.eV, mass: float = u.electron_mass, **kwargs):
+        return cls.from_frequency_and_mass(omega=energy_spacing / u.hbar, mass=mass, **kwargs)
+
+    def __call__(self, *, r, **kwargs):
+        rr = np.asarray(r) - self.center
+        inside = 0.5 * self.spring_constant * (rr ** 2)
+        if self.cutoff_distance is None:
+            return inside
+        return np.where(np.abs(rr) <= self.cutoff_distance, inside, 0.5 * self.spring_constant * self.cutoff_distance ** 2)
+
+    def omega(self, mass: float) -> float:
+        return np.sqrt(self.spring_constant / mass)
+
+    def __repr__(self):
+        return f"HarmonicOscillator(spring_constant={self.spring_constant}, center={self.center})"
+
+
 class SoftCoulombPotential(PotentialEnergy):
     """k q Q / sqrt(r^2 + s^2) (``potentials/static.py:54-91``)."""
 
@@ -221,6 +256,12 @@ class SineWave(UniformLinearlyPolarizedElectricPotential):
         self.omega = omega
         self.phase = phase % u.twopi
         self.amplitude = amplitude
+
+    @classmethod
+    def from_photon_energy(cls, photon_energy: float, amplitude: float = 1 * u.atomic_electric_field, phase=0,
+                           **kwargs):
+        """(pulses.py:316-337)"""
+        return cls(photon_energy / u.hbar, amplitude=amplitude, phase=phase, **kwargs)
 
     @property
     def period(self):

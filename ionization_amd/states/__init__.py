@@ -200,6 +200,50 @@ class NumericOneDState(QuantumState):
         return f"NumericOneDState(energy={self.energy / u.eV:.6g} eV)"
 
 
+class QHOState(QuantumState):
+    """
+    Quantum harmonic oscillator eigenstate n of spring constant k and mass m
+    (``ionization/states/one_d.py:130-300``): psi_n(x) =
+    (m w / (pi hbar))^(1/4) / sqrt(2^n n!) H_n(sqrt(m w / hbar) x) exp(-m w x^2 / (2 hbar)),
+    w = sqrt(k / m).
+    """
+
+    def __init__(self, spring_constant: float, mass: float = u.electron_mass, n: int = 0, amplitude=1):
+        self.n = int(n)
+        self.spring_constant = spring_constant
+        self.mass = mass
+        super().__init__(amplitude=amplitude)
+
+    @classmethod
+    def from_omega_and_mass(cls, omega: float, mass: float = u.electron_mass, n: int = 0, amplitude=1):
+        return cls(spring_constant=mass * (omega ** 2), mass=mass, n=n, amplitude=amplitude)
+
+    @classmethod
+    def from_potential(cls, potential, mass: float, n: int = 0, amplitude=1):
+        return cls(spring_constant=potential.spring_constant, mass=mass, n=n, amplitude=amplitude)
+
+    @property
+    def omega(self):
+        return np.sqrt(self.spring_constant / self.mass)
+
+    @property
+    def energy(self):
+        return u.hbar * self.omega * (self.n + 0.5)
+
+    @property
+    def tuple(self):
+        return self.n, self.mass, self.omega
+
+    def __call__(self, x):
+        mw = self.mass * self.omega / u.hbar
+        norm = (mw / u.pi) ** 0.25 / (np.float64(2.0 ** (self.n / 2)) * np.sqrt(np.float64(special.factorial(self.n))))
+        x = np.asarray(x, dtype=np.float64)
+        return (norm * np.exp(-mw * x ** 2 / 2) * special.eval_hermite(self.n, np.sqrt(mw) * x)).astype(np.complex128)
+
+    def __repr__(self):
+        return f"QHOState(n={self.n}, energy={self.energy / u.eV:.6g} eV)"
+
+
 class GroundStateLabel(QuantumState):
     """Key for the lowest eigenstate of a line potential with no analytic counterpart."""
 

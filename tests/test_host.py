@@ -315,3 +315,26 @@ def test_velocity_gauge_descriptor_and_limits():
         engine.compile_sh(r_bound=10 * u.bohr_radius, r_points=20, l_bound=4, potential=physics.coulomb,
                           field=lambda t: 0.0 * t, mask=None, times=np.arange(3) * u.asec, time_step=u.asec,
                           g0=np.zeros((4, 20)), gauge="magnetic")
+
+
+def test_harmonic_oscillator_and_qho_states():
+    """HarmonicOscillator (potentials/static.py:113-233) and QHOState (states/one_d.py:130-300):
+    k = m w^2 from the energy spacing; the eigenstates are orthonormal and are eigenvectors of
+    the discretised K + V with energies hbar w (n + 1/2) to the grid's accuracy."""
+    m = u.electron_mass
+    qho = potentials.HarmonicOscillator.from_energy_spacing_and_mass(energy_spacing=0.1 * u.eV, mass=m)
+    assert np.isclose(qho.omega(m) * u.hbar, 0.1 * u.eV, rtol=1e-13)
+    z = np.linspace(-30 * u.nm, 30 * u.nm, 6001)
+    dz = z[1] - z[0]
+    vecs = [states.QHOState.from_potential(qho, n=n, mass=m)(z) for n in range(4)]
+    gram = np.array([[np.sum(np.conj(a) * b) * dz for b in vecs] for a in vecs])
+    assert np.allclose(gram, np.eye(4), atol=1e-9)
+    pref = -(u.hbar ** 2) / (2 * m * dz ** 2)
+    for n, v in enumerate(vecs):
+        hv = (-2 * pref + qho(r=z)) * v
+        hv[1:] += pref * v[:-1]
+        hv[:-1] += pref * v[1:]
+        e = (np.sum(np.conj(v) * hv) * dz).real
+        assert np.isclose(e, states.QHOState.from_potential(qho, n=n, mass=m).energy, rtol=1e-4)
+    sine = potentials.SineWave.from_photon_energy(photon_energy=0.1 * u.eV)
+    assert np.isclose(sine.omega * u.hbar, 0.1 * u.eV, rtol=1e-13)
