@@ -135,3 +135,54 @@ def test_scan_batch_absorbs_and_is_batch_independent(device_ctx):
     alone = device_ctx.download_sh(s).g
     device_ctx.remove(engine.KIND_SH, s)
     assert np.array_equal(first, alone)
+
+
+def test_sh_velocity_gauge_200x1000_8000_steps_unitary_and_field_free_stationary(device_ctx):
+    """The velocity gauge's factors are real rotations and the CN step is unitary: without a mask
+    the norm stays put with the pulse; without a field the numeric 1s is stationary."""
+    from ionization_amd import mesh
+
+    spec = workloads.sh_single_spec("vu", store_data_every=500, numeric_eigenstate_max_energy=0.0,
+                                    operators=mesh.SphericalHarmonicVelocityGaugeOperators())
+    spec.mask = potentials.NoMask()
+    sim = spec.to_sim()
+    comp = spec.evolution_method.compile(sim)
+    assert comp.n_steps == 8000 and comp.gauge == "velocity"
+    r = _run_sh(device_ctx, comp)
+    print(f"velocity norm drift {np.max(np.abs(r.norms - r.norms[0])):.2e}", flush=True)
+    assert np.max(np.abs(r.norms - r.norms[0])) <= 1e-9
+    assert np.abs(r.ips[-1, 0]) < 1 - 1e-8
+    spec = workloads.sh_single_spec("vs", store_data_every=1000, numeric_eigenstate_max_energy=0.0,
+                                    operators=mesh.SphericalHarmonicVelocityGaugeOperators())
+    spec.electric_potential = potentials.NoElectricPotential()
+    spec.mask = potentials.NoMask()
+    r = _run_sh(device_ctx, spec.evolution_method.compile(spec.to_sim()))
+    assert np.max(np.abs(np.abs(r.ips[:, 0]) ** 2 - 1)) <= 1e-9
+
+
+def test_sh_adi_200x1000_8000_steps_invariant_with_field(device_ctx):
+    """
+    SH ADI: one step is (1 + i tau H0)^-1 C_int (1 - i tau H0) = S^-1 C_int C_0 S with S = 1 + i tau H0
+    and unit-modulus Cayley factors, so with a field and no mask |S g| is conserved exactly while
+    |g| is not.
+    """
+    from ionization_amd import mesh
+    from oracle import mesh_cpu, physics
+
+    spec = workloads.sh_single_spec("adi", store_data_every=8000, numeric_eigenstate_max_energy=0.0,
+                                    evolution_method=mesh.AlternatingDirectionImplicit())
+    spec.mask = potentials.NoMask()
+    sim = spec.to_sim()
+    g0 = np.array(sim.mesh.g)
+    comp = spec.evolution_method.compile(sim)
+    assert comp.n_steps == 8000 and comp.method == "adi"
+    r = _run_sh(device_ctx, comp)
+    orc = mesh_cpu.SHOracle(r_bound=spec.r_bound, r_points=spec.r_points, l_bound=spec.l_bound,
+                            potential=physics.coulomb, field=lambda t: 0.0 * t)
+    S = mesh_cpu.add_to_diagonal((1j * spec.time_step / (2 * u.hbar)) * orc.h0, value=1)
+    s0 = np.linalg.norm(S.dot(g0.reshape(-1)))
+    sN = np.linalg.norm(S.dot(r.g.reshape(-1)))
+    print(f"ADI |S g| drift {abs(sN - s0) / s0:.2e}, |g| drift {np.max(np.abs(r.norms - r.norms[0])):.2e}",
+          flush=True)
+    assert abs(sN - s0) <= 1e-9 * s0
+    assert np.abs(r.ips[-1, 0]) < 1 - 1e-8
