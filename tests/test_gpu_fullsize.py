@@ -186,3 +186,24 @@ def test_sh_adi_200x1000_8000_steps_invariant_with_field(device_ctx):
           flush=True)
     assert abs(sN - s0) <= 1e-9 * s0
     assert np.abs(r.ips[-1, 0]) < 1 - 1e-8
+
+
+@pytest.mark.parametrize("velocity", [False, True])
+def test_line_split_operator_8192x4000_unitary_without_mask(device_ctx, velocity):
+    """The Line split operator's factors (a phase, or real z-pair rotations) and its CN step are
+    unitary: without a mask the norm of the 8192-point, 4000-step run stays put."""
+    from ionization_amd import mesh
+
+    ops = mesh.LineVelocityGaugeOperators() if velocity else mesh.LineLengthGaugeOperators()
+    spec = workloads.line_cn_spec(z_points=8192, n_steps=4000, store_data_every=100, ctx=device_ctx, operators=ops,
+                                  evolution_method=mesh.SplitInteractionOperator())
+    spec.mask = potentials.NoMask()
+    comp = spec.evolution_method.compile(spec.to_sim())
+    assert comp.method == (2 if velocity else 1) and comp.n_steps == 4000
+    s = device_ctx.add_line(comp)
+    device_ctx.line_run(comp.n_steps)
+    g, norms, _ = device_ctx.download_line(s)
+    device_ctx.remove(engine.KIND_LINE, s)
+    print(f"Line SO {'velocity' if velocity else 'length'} norm drift {np.max(np.abs(norms - norms[0])):.2e}",
+          flush=True)
+    assert np.max(np.abs(norms - norms[0])) <= 1e-9
