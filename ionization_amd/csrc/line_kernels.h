@@ -68,10 +68,13 @@ __device__ __forceinline__ void line_phase(double2 (&x)[M], const int k, const d
 // on the z-pairs (j, j + 1), j + 1 < nz, of even j then odd j (rev: odd, then
 // even). With M even the even pairs lie inside a thread; of the odd pairs the
 // one across a chunk edge uses the neighbouring thread's row through LDS.
+// Forward (odd pairs last), each thread's edge rotations also produce the
+// neighbouring chunks' final edge rows: xl / xr, when given, receive them (what
+// chunk_neighbours would read), so the CN solve needs no exchange of its own.
 template <int M>
 __device__ __forceinline__ void line_vel_pairs(double2 (&x)[M], const int k, const int P,
                                                double2* __restrict__ lds, const double a, const int nz,
-                                               const bool rev) {
+                                               const bool rev, double2* xl = nullptr, double2* xr = nullptr) {
     double sn, cs;  // one angle for the whole line (uniform across the block)
     if (fabs(a) <= kSmallAngle) tdse_sincos_small(a, &sn, &cs);
     else tdse_sincos(a, &sn, &cs);
@@ -94,14 +97,12 @@ __device__ __forceinline__ void line_vel_pairs(double2 (&x)[M], const int k, con
         lds[k] = x[M - 1];
         lds[P + k] = x[0];
         __syncthreads();
-        if (k + 1 < P && (k + 1) * M < nz) {
-            double2 r = lds[P + k + 1];
-            rot(x[M - 1], r);
-        }
-        if (k > 0 && k * M < nz) {
-            double2 l = lds[k - 1];
-            rot(l, x[0]);
-        }
+        double2 r = (k + 1 < P) ? lds[P + k + 1] : cmk(0.0, 0.0);
+        if (k + 1 < P && (k + 1) * M < nz) rot(x[M - 1], r);
+        double2 l = (k > 0) ? lds[k - 1] : cmk(0.0, 0.0);
+        if (k > 0 && k * M < nz) rot(l, x[0]);
+        if (xl) *xl = l;
+        if (xr) *xr = r;
     };
     if (rev) {
         odd();
@@ -194,11 +195,13 @@ __global__ void __launch_bounds__(MAXT) k_line_steps(LineSim* __restrict__ sims,
         const double fs = ld1(S.fs, n);
         // split operator: the interaction factors before the CN half
         if constexpr (METHOD == LINE_SIO_LEN) line_phase<M>(x, k, w.y * fs, S.zq0, S.dzq);
-        if constexpr (METHOD == LINE_SIO_VEL) line_vel_pairs<M>(x, k, P, lds, w.y * fs * S.vq, S.n, false);
+        double2 xl, xr;
+        if constexpr (METHOD == LINE_SIO_VEL)  // the odd pairs' edge rotations give the neighbours' edge rows
+            line_vel_pairs<M>(x, k, P, lds, w.y * fs * S.vq, S.n, false, &xl, &xr);
+        else
+            chunk_neighbours<M>(x, k, P, lds, xl, xr, true);
         // the interaction sits in the CN diagonal only for the ADI step
         const double f = (METHOD == LINE_ADI) ? fs : 0.0;
-        double2 xl, xr;
-        chunk_neighbours<M>(x, k, P, lds, xl, xr, true);
         if constexpr (IMAG) {
             // w = i s with s real: solve with H~ = s H by the low-register solver,
             // which re-reads the (L2-resident) tables instead of holding them
