@@ -632,3 +632,36 @@ def test_sh_adi_field_free_stationary_and_its_invariant():
     s0, sN = np.linalg.norm(S.dot(g2.reshape(-1))), np.linalg.norm(S.dot(gN.reshape(-1)))
     assert abs(sN - s0) < 1e-12 * s0
     assert abs(norms2[-1] - norms2[0]) > 1e-6 * norms2[0]  # while the plain norm is not conserved
+
+
+def test_line_velocity_split_step_matches_dense_composition():
+    """Line SIO, velocity gauge: O E . CN(K + V) . E O order reversed around the solve and the mask,
+    with E / O = expm(-i tau A h_even / h_odd) built densely from the definition (mesh_operators.py:
+    352-427, evolution_methods.py:113-121) and A = -simps(E, t_0 .. t_{n+1})."""
+    n = 11
+    maskf = lambda z: physics.radial_cosine_mask(z, 10 * u.bohr_radius, 20 * u.bohr_radius)
+    fld = lambda t: 30.0 * u.atomic_electric_field * np.cos(t / (40 * u.asec))
+    o = mesh_cpu.LineSIOOracle(z_bound=20 * u.bohr_radius, z_points=n,
+                               potential=lambda z: physics.soft_coulomb(z, np.sqrt(2) * u.bohr_radius),
+                               field=fld, mask=maskf, test_mass=u.electron_mass, gauge="velocity")
+    rng = np.random.default_rng(8)
+    g = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ts = np.arange(5) * 3 * u.asec
+    dt = ts[-1] - ts[-2]
+    out = o.evolve_sio(g, dt, ts[-1], ts)
+    tau = dt / (2 * u.hbar)
+    A = -mesh_cpu.simps_old(fld(ts), ts)
+    pref = A * 1j * u.hbar * (u.electron_charge / u.electron_mass) / (2 * o.delta_z)
+    H = np.diag(np.full(n - 1, pref), 1) - np.diag(np.full(n - 1, pref), -1)
+    parts = []
+    for par in (0, 1):
+        Pm = np.zeros_like(H)
+        for k in range(par, n - 1, 2):
+            Pm[k, k + 1], Pm[k + 1, k] = H[k, k + 1], H[k + 1, k]
+        parts.append(sl.expm(-1j * tau * Pm))
+    Ev, Od = parts
+    H0 = o.h_internal.toarray()
+    cn = np.linalg.solve(np.eye(n) + 1j * tau * H0, np.eye(n) - 1j * tau * H0)
+    assert abs(tau * pref) > 0.05  # the factors are far from the identity
+    ref = (Ev @ Od @ cn @ Od @ Ev @ g) * maskf(o.z)
+    assert rel_l2(out, ref) < 1e-12
