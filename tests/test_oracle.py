@@ -665,3 +665,46 @@ def test_line_velocity_split_step_matches_dense_composition():
     assert abs(tau * pref) > 0.05  # the factors are far from the identity
     ref = (Ev @ Od @ cn @ Od @ Ev @ g) * maskf(o.z)
     assert rel_l2(out, ref) < 1e-12
+
+
+def test_velocity_sh_step_matches_dense_composition():
+    """One velocity-gauge SIO step = M . h1E h1O . EE EO OE OO (reversed) . CN . OO OE EO EE . h1O h1E
+    in application order (evolution_methods.py:113-121), every factor expm(-i tau part) of the
+    dense h1 / h2 parts and A = -simps(E, t_0 .. t_{n+1})."""
+    L, n = 4, 5
+    rb = 10 * u.bohr_radius
+    maskf = lambda r: physics.radial_cosine_mask(r, 0.5 * rb, rb)
+    fld = lambda t: 20.0 * u.atomic_electric_field * np.cos(t / (30 * u.asec))
+    o = _tiny_vel(n, L, field=fld, mask=maskf)
+    rng = np.random.default_rng(9)
+    g = rng.standard_normal((L, n)) + 1j * rng.standard_normal((L, n))
+    ts = np.arange(4) * 2 * u.asec
+    dt = ts[-1] - ts[-2]
+    out = o.evolve_velocity(g, dt, ts)
+    tau = dt / (2 * u.hbar)
+    A = -mesh_cpu.simps_old(fld(ts), ts)
+    Q = np.zeros((L * n, L * n))  # r-order index l n + j from l-order index j L + l
+    for l in range(L):
+        for j in range(n):
+            Q[l * n + j, j * L + l] = 1
+    H1 = Q @ (A * o.h1).toarray() @ Q.T  # both in r-order
+    H2 = (A * o.h2).toarray()
+    ex = lambda P_: sl.expm(-1j * tau * P_)
+    h1 = [ex(_masked_part((A * o.h1).toarray(), lambda a, b, p=p: min(a, b) % 2 == p)) for p in (0, 1)]
+    h1 = [Q @ M @ Q.T for M in h1]
+    h2 = [ex(_masked_part(H2, lambda a, b, lp=lp, rp=rp: min(a // n, b // n) % 2 == lp and min(a % n, b % n) % 2 == rp))
+          for lp, rp in [(0, 0), (0, 1), (1, 0), (1, 1)]]
+    split = [h1[0], h1[1], *h2]  # application order
+    H0 = _dense_h0_closed_form(o)
+    N = L * n
+    cn = np.linalg.solve(np.eye(N) + 1j * tau * H0, np.eye(N) - 1j * tau * H0)
+    v = g.reshape(-1)
+    for M in split:
+        v = M @ v
+    v = cn @ v
+    for M in reversed(split):
+        v = M @ v
+    ref = v.reshape(L, n) * maskf(o.r_mesh)
+    assert np.abs(tau * A * u.hbar * u.electron_charge / (u.electron_mass * o.delta_r)) > 0.05
+    assert rel_l2(out, ref) < 1e-12
+    assert np.allclose(H1 + H2, (H1 + H2).conj().T)  # the interaction is Hermitian
