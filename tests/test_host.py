@@ -338,3 +338,38 @@ def test_harmonic_oscillator_and_qho_states():
         assert np.isclose(e, states.QHOState.from_potential(qho, n=n, mass=m).energy, rtol=1e-4)
     sine = potentials.SineWave.from_photon_energy(photon_energy=0.1 * u.eV)
     assert np.isclose(sine.omega * u.hbar, 0.1 * u.eV, rtol=1e-13)
+
+
+def test_simps_cumulative_small_prefixes():
+    """One sample integrates to 0, two samples are the trapezoid (old SciPy simps on N = 1, 2)."""
+    t = np.array([0.0, 0.5, 1.25])
+    y = np.array([1.0, 3.0, 2.0])
+    c = potentials.simps_cumulative(y, t)
+    assert c[0] == 0.0
+    assert np.isclose(c[1], 0.5 * 0.5 * (1.0 + 3.0), rtol=1e-15)
+    assert np.isclose(c[2], mesh_cpu.simps_old(y, t), rtol=1e-14)
+
+
+def test_unsupported_method_combinations_raise():
+    """The device path refuses combinations it does not implement instead of running elsewhere."""
+    kw = dict(r_bound=10 * u.bohr_radius, r_points=20, l_bound=4, potential=physics.coulomb, field=lambda t: 0.0 * t,
+              mask=None, times=np.arange(3) * u.asec, time_step=u.asec, g0=np.zeros((4, 20)))
+    with pytest.raises(exceptions.DeviceError):
+        engine.compile_sh(method="adi", gauge="velocity", **kw)
+    with pytest.raises(exceptions.DeviceError):
+        engine.compile_sh(method="crank", **kw)
+    lkw = dict(z_bound=10 * u.bohr_radius, z_points=64, potential=lambda z: 0.0 * z, field=lambda t: 0.0 * t,
+               mask=None, times=np.arange(3) * u.asec, g0=np.zeros(64))
+    with pytest.raises(exceptions.DeviceError):
+        engine.compile_line(method="sio", time_steps=np.array([1j, 1j]) * u.asec, **lkw)
+    with pytest.raises(exceptions.DeviceError):
+        engine.compile_line(method="adi", gauge="velocity", **lkw)
+    spec = mesh.LineSpecification("l", initial_state=states.NumericOneDState(g=np.ones(64), energy=0.0),
+                                  z_bound=10 * u.bohr_radius, z_points=64, operators=mesh.LineVelocityGaugeOperators(),
+                                  evolution_method=mesh.AlternatingDirectionImplicit())
+    with pytest.raises(NotImplementedError):
+        spec.evolution_method.compile(spec.to_sim())
+    sh = _small_sh_spec(operators=mesh.SphericalHarmonicVelocityGaugeOperators(),
+                        evolution_method=mesh.AlternatingDirectionImplicit())
+    with pytest.raises(NotImplementedError):
+        sh.evolution_method.compile(sh.to_sim())
