@@ -146,7 +146,7 @@ def time_sh_adi(params, store_every, steps, warmup):
     return out
 
 
-def time_reference_specs():
+def time_reference_specs(cpu_seconds=0.0):
     """
     The specifications of the reference's own benchmark (benchmarks/benchmark_mesh_evolution.py:
     43-166) that are on the device path, built and run one at a time through the drop-in API
@@ -185,6 +185,24 @@ def time_reference_specs():
                                             evolution_method=mesh.SplitInteractionOperator(), **sh_kw),
     ]
     out = {}
+    if cpu_seconds > 0:
+        # the oracle (a restatement of the reference's per-step sparse builds, dia products and Thomas
+        # solves) on the SH split-operator spec for a bounded number of steps: the CPU side of the table
+        from oracle import mesh_cpu
+
+        sp = specs[5]
+        sim0 = sp.to_sim()
+        orc = mesh_cpu.SHOracle(r_bound=sp.r_bound, r_points=sp.r_points, l_bound=sp.l_bound,
+                                potential=lambda r: sp.internal_potential(r=r, test_charge=sp.test_charge),
+                                field=lambda t: sp.electric_potential.get_electric_field_amplitude(t))
+        g, steps, t0 = np.array(sim0.mesh.g), 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds and steps < len(sim0.times) - 1:
+            g = orc.evolve(g, sim0.times[steps + 1] - sim0.times[steps], sim0.times[steps + 1], sp.time_step)
+            steps += 1
+        out["cpu_port"] = {"spec": sp.name, "mesh_point_steps_per_s": g.size * steps / (time.perf_counter() - t0),
+                           "sample": f"{steps} steps, one core, evolve only"}
+        log(f"reference benchmark spec {sp.name} on the CPU oracle: "
+            f"{out['cpu_port']['mesh_point_steps_per_s'] / 1e6:.2f} M mesh-point-steps/s")
     for spec in specs:
         w0, c0 = time.perf_counter(), time.process_time()
         sim = spec.to_sim()
@@ -719,7 +737,7 @@ def main():
 
     vel = time_velocity(params, args.store_every, args.steps, args.warmup) if args.velocity else None
     sh_adi = time_sh_adi(params, args.store_every, args.steps, args.warmup) if args.sh_adi else None
-    ref_specs = time_reference_specs() if args.reference_specs else None
+    ref_specs = time_reference_specs(min(args.cpu_seconds, 5.0)) if args.reference_specs else None
     cyl = time_cyl(args.cyl_sims, args.seed, args.cyl_steps, 10) if args.cyl_sims > 0 else None
     line = time_line(args.line_batch, min(args.cpu_seconds, 5.0)) if args.line_batch > 0 else None
     served = (time_serve(args.serve_requests, args.serve_rate, args.seed, args.serve_pool_gb)
