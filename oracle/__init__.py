@@ -27,10 +27,22 @@ Pinning (what checks each part against something other than itself):
   * SH / Line evolution   — field-free eigenstate stationarity at 1e-14
                             (tests/mesh/test_sims.py:30-82).
   * mask, pulses          — identities of tests/potentials/test_masks.py:82-108
-                            and tests/potentials/test_pulses.py:169-219.
+                            and tests/potentials/test_pulses.py:169-219, plus
+                            closed forms: the mask's interior values (midpoint
+                            2^(-1/(2s)), complementary points); the Gaussian
+                            pulse's envelope, fluence integral and Fourier
+                            transform; the sinc pulse's flat band on
+                            [w_min, w_min + 2 pi / pw]; the trapezoidal
+                            window's half points and mirror symmetry.
+  * potentials, states    — Coulomb = -1 hartree at one bohr, the softened form
+                            in atomic units; hydrogen R_nl values at the origin,
+                            R_21 in closed form, orthogonality, normalisation.
+  * time grid, data times — known answers of the int() truncation and cadence.
   * eigenbasis            — ARPACK ``eigsh`` (the reference's routine) against
                             LAPACK ``eigh_tridiagonal``; hydrogen energies.
-  * unit constants        — parity unpinned (no reference test fixes
-                            simulacra's numeric values; one CODATA table is
-                            shared by oracle and device path).
+  * unit constants        — the derived atomic units (bohr, hartree, atomic
+                            time and field) against their definitions; the
+                            primary constants are parity unpinned (no reference
+                            test fixes simulacra's numeric values; one CODATA
+                            table is shared by oracle and device path).
 """

@@ -341,6 +341,176 @@ def test_hydrogen_radial_normalised(n, l):
     assert abs(val - 1) < 1e-8
 
 
+# ------------------------------------------------------------------ closed forms of the host inputs
+#
+# Each check below is a value or an invariant that the mathematics fixes independently of the
+# oracle's own formula, chosen so that a plausible slip (a power inverted, a width or carrier
+# frequency off by a factor, a ramp edge swapped) fails it.
+
+
+@pytest.mark.parametrize("s", [1.0, 2.0, 3.0, 8.0, 16.0])
+def test_radial_cosine_mask_interior_closed_forms(s):
+    """masks.py:76-89: m = |cos(pi x / 2)|^(1/s) on x = (r - r_in) / (r_out - r_in) in [0, 1)."""
+    r_in, r_out = 3.0, 11.0
+    at = lambda x: float(physics.radial_cosine_mask(r_in + x * (r_out - r_in), r_in, r_out, s))
+    # midpoint: cos(pi / 4) = 2^(-1/2), so m = 2^(-1/(2 s))
+    assert at(0.5) == pytest.approx(2.0 ** (-1.0 / (2 * s)), rel=1e-14)
+    # quarter point: cos(pi / 8) = sqrt(2 + sqrt 2) / 2
+    assert at(0.25) == pytest.approx((np.sqrt(2 + np.sqrt(2)) / 2) ** (1 / s), rel=1e-14)
+    # complementary points: m(x)^(2s) + m(1 - x)^(2s) = cos^2 + sin^2 = 1
+    for x in [0.1, 0.3, 0.45, 0.7]:
+        assert at(x) ** (2 * s) + at(1 - x) ** (2 * s) == pytest.approx(1.0, abs=1e-13)
+    # the mask is 1 up to and including r_in, 0 from r_out on, and decreasing between
+    r = np.linspace(0, 14, 281)
+    m = physics.radial_cosine_mask(r, r_in, r_out, s)
+    assert np.all(m[r <= r_in] == 1.0) and np.all(m[r >= r_out] == 0.0)
+    assert np.all(np.diff(m) <= 0)
+
+
+def _fourier(t, e, omegas):
+    """int e(t) exp(-i w t) dt by the trapezoid rule on a uniform grid (spectrally accurate for smooth,
+    decaying integrands)."""
+    dt = t[1] - t[0]
+    return np.array([np.sum(e * np.exp(-1j * w * t)) * dt for w in omegas])
+
+
+@pytest.mark.parametrize("pw, amp, phase", [(100 * u.asec, 0.01, 0.0), (300 * u.asec, 0.05, 0.4),
+                                            (500 * u.asec, 0.1, 2.5)])
+def test_gaussian_pulse_closed_forms(pw, amp, phase):
+    """pulses.py:1016-1175, 1361-1372: E = A exp(-tau^2 / (2 pw^2)) cos(w_c tau + phi),
+    w_c = w_min + pi / pw (the carrier of a sinc pulse of the same width), fluence
+    F = eps0 c A^2 pw sqrt(pi) / 2 (1 + exp(-w_c^2 pw^2) cos 2 phi)."""
+    A = amp * u.atomic_electric_field
+    f = lambda t: physics.gaussian_pulse_field(t, pulse_width=pw, amplitude=A, phase=phase)
+    w_min = u.twopi * 30 * u.THz
+    wc = w_min + u.pi / pw
+    # envelope: exp(-1/2) one width from the centre, exp(-2) at two, where the carrier is known
+    for k in (1.0, 2.0):
+        tau = k * pw
+        assert f(tau) == pytest.approx(A * np.exp(-k * k / 2) * np.cos(wc * tau + phase), rel=1e-12, abs=1e-15 * A)
+    # fluence of the amplitude-specified pulse, against the Gaussian integral
+    t = np.linspace(-14 * pw, 14 * pw, 20001)
+    fl = u.epsilon_0 * u.c * np.sum(f(t) ** 2) * (t[1] - t[0])
+    closed = u.epsilon_0 * u.c * A ** 2 * pw * np.sqrt(u.pi) / 2 * (1 + np.exp(-(wc * pw) ** 2) * np.cos(2 * phase))
+    assert fl == pytest.approx(closed, rel=1e-10)
+    g = potentials.GaussianPulse.from_amplitude(pulse_width=pw, amplitude=A, phase=phase)
+    assert u.epsilon_0 * u.c * A ** 2 * pw * np.sqrt(u.pi) / 2 == pytest.approx(g.fluence, rel=1e-12)
+    # spectrum: (A / 2) pw sqrt(2 pi) [e^{i phi} G(w - w_c) + e^{-i phi} G(w + w_c)], G(v) = exp(-v^2 pw^2 / 2);
+    # the peak is at w_c
+    omegas = wc + np.array([-1.5, -0.5, 0.0, 0.7, 2.0]) / pw
+    got = _fourier(t, f(t), omegas)
+    G = lambda v: np.exp(-(v * pw) ** 2 / 2)
+    want = (A / 2) * pw * np.sqrt(u.twopi) * (np.exp(1j * phase) * G(omegas - wc) + np.exp(-1j * phase) * G(omegas + wc))
+    assert np.allclose(got, want, rtol=0, atol=1e-10 * abs(want[2]))
+
+
+@pytest.mark.parametrize("pw, phase", [(50 * u.asec, 0.0), (200 * u.asec, 1.1), (800 * u.asec, 3.0)])
+def test_sinc_pulse_spectrum_is_flat_on_its_band(pw, phase):
+    """pulses.py:646-695, 929-940: the sinc pulse's spectrum is flat, (A / 2)(2 pi / dw) e^{i phi}, on
+    [w_min, w_min + 2 pi / pw] and zero above it, with A^2 = F dw / (pi eps0 c).
+    A wide Gaussian window W makes the numerical transform converge; it blurs the band edges by only 1 / W."""
+    F = 1 * u.Jcm2
+    f = lambda t: physics.sinc_pulse_field(t, pulse_width=pw, fluence=F, phase=phase)
+    dw = u.twopi / pw
+    w_min = u.twopi * 30 * u.THz
+    A = np.sqrt(F * dw / (u.pi * u.epsilon_0 * u.c))
+    W = 40 * pw
+    t = np.arange(-12 * W, 12 * W, pw / 24)
+    e = f(t) * np.exp(-0.5 * (t / W) ** 2)
+    inside = w_min + np.array([0.2, 0.35, 0.5, 0.65, 0.8]) * dw
+    outside = w_min + np.array([1.2, 1.6, 3.0]) * dw
+    flat = (A / 2) * (u.twopi / dw) * np.exp(1j * phase)
+    assert np.allclose(_fourier(t, e, inside), flat, rtol=0, atol=1e-9 * abs(flat))
+    assert np.all(np.abs(_fourier(t, e, outside)) < 1e-9 * abs(flat))
+    # the mirror band at negative frequencies carries the conjugate phase
+    assert np.allclose(_fourier(t, e, -inside), np.conj(flat), rtol=0, atol=1e-9 * abs(flat))
+    # envelope zeros one and two pulse widths from the centre, 2 / pi at half a width
+    tau = np.array([pw, 2 * pw])
+    wc = w_min + dw / 2
+    assert np.all(np.abs(f(tau)) < 1e-15 * A)
+    assert f(pw / 2) == pytest.approx(A * (2 / u.pi) * np.cos(wc * pw / 2 + phase), rel=1e-12, abs=1e-15 * A)
+
+
+def test_smoothed_trapezoidal_window_closed_forms():
+    """windows.py:203-225: sin^2 rise over time_front, plateau, cos^2 fall, then 0."""
+    tf, tp, omega, amp = 120 * u.asec, 70 * u.asec, 1e16, 2.0
+    w = lambda t: physics.sine_wave_field(t, omega=omega, amplitude=amp, time_front=tf, time_plateau=tp) / (
+        np.sin(omega * np.asarray(t)) * amp)
+    end = 2 * tf + tp
+    assert w(tf / 2) == pytest.approx(0.5, rel=1e-13)
+    assert w(tf + tp + tf / 2) == pytest.approx(0.5, rel=1e-13)
+    assert w(tf / 4) == pytest.approx((2 - np.sqrt(2)) / 4, rel=1e-12)  # sin^2(pi / 8)
+    assert w(tf) == 1.0 and w(tf + tp) == 1.0 and w(tf + 0.5 * tp) == 1.0
+    assert w(end + 3 * u.asec) == 0.0
+    # mirror symmetry about the plateau's centre on [0, end]
+    t = np.linspace(0.01 * end, 0.99 * end, 173)
+    t = t[np.abs(np.sin(omega * t)) > 1e-3]
+    t = t[np.abs(np.sin(omega * (end - t))) > 1e-3]
+    assert np.allclose(w(t), w(end - t), rtol=1e-12, atol=1e-14)
+    # without a window the field is A sin(w t + phi): zero at t = (k pi - phi) / w, A at a quarter period
+    phi = 0.3
+    f = lambda t: physics.sine_wave_field(t, omega=omega, amplitude=amp, phase=phi)
+    assert abs(f((u.pi - phi) / omega)) < 1e-14 * amp
+    assert f((u.pi / 2 - phi) / omega) == pytest.approx(amp, rel=1e-14)
+
+
+def test_derived_atomic_units_follow_their_definitions():
+    """The unit table's derived atomic units against their definitions from the primary constants
+    (the primary constants themselves are parity-unpinned: simulacra.units is absent)."""
+    a0 = 4 * u.pi * u.epsilon_0 * u.hbar ** 2 / (u.electron_mass * u.proton_charge ** 2)
+    assert u.bohr_radius == pytest.approx(a0, rel=1e-9)
+    e_h = u.hbar ** 2 / (u.electron_mass * u.bohr_radius ** 2)
+    assert u.hartree == pytest.approx(e_h, rel=1e-9)
+    assert u.hartree == 2 * u.rydberg
+    assert u.atomic_time == pytest.approx(u.hbar / e_h, rel=1e-9)
+    assert u.atomic_electric_field == pytest.approx(e_h / (u.proton_charge * u.bohr_radius), rel=1e-9)
+    assert u.coulomb_constant * u.proton_charge ** 2 / u.bohr_radius == pytest.approx(e_h, rel=1e-9)
+    assert u.electron_mass_reduced == pytest.approx(1 / (1 / u.electron_mass + 1 / u.proton_mass), rel=1e-15)
+    assert u.Jcm2 == 1 / (1e-2) ** 2 and u.asec == 1e-18 and u.THz == 1e12 and u.eV == u.proton_charge
+
+
+def test_coulomb_and_soft_coulomb_closed_forms():
+    """static.py:26-38, 75-91: -1 hartree one bohr from a proton; -1/sqrt(z^2 + s^2) hartree softened."""
+    assert physics.coulomb(u.bohr_radius) == pytest.approx(-u.hartree, rel=1e-9)
+    assert physics.coulomb(3 * u.bohr_radius, charge=2 * u.proton_charge) == pytest.approx(-2 * u.hartree / 3,
+                                                                                            rel=1e-9)
+    s = np.sqrt(2) * u.bohr_radius
+    for z in [0.0, 1.0, 5.0]:
+        want = -u.hartree / np.sqrt(z * z + 2)
+        assert physics.soft_coulomb(z * u.bohr_radius, s) == pytest.approx(want, rel=1e-9)
+
+
+def test_hydrogen_radial_values_and_orthogonality():
+    """three_d.py:375-391 against the textbook R_10(0) = 2 a0^-3/2, R_20(0) = a0^-3/2 / sqrt 2,
+    R_21(r) = r e^{-r/2} / (2 sqrt 6) (a0 units), and <R_10|R_20> = 0."""
+    a = u.bohr_radius
+    assert physics.hydrogen_radial(1, 0, 0.0) == pytest.approx(2 * a ** -1.5, rel=1e-13)
+    assert physics.hydrogen_radial(2, 0, 0.0) == pytest.approx(a ** -1.5 / np.sqrt(2), rel=1e-13)
+    for x in [0.5, 2.0, 7.0]:
+        assert physics.hydrogen_radial(2, 1, x * a) == pytest.approx(
+            a ** -1.5 * x * np.exp(-x / 2) / (2 * np.sqrt(6)), rel=1e-13)
+    f = lambda x: physics.hydrogen_radial(1, 0, x * a) * physics.hydrogen_radial(2, 0, x * a) * (x * a) ** 2 * a
+    val, _ = integrate.quad(f, 0, 200)
+    assert abs(val) < 1e-10
+
+
+def test_time_grid_truncates_like_int():
+    """sims.py:198-205: int() truncation of (tf - t0) / dt, so 0.3 / 0.1 = 2.999... gives 3 points."""
+    assert len(mesh_cpu.time_grid(0.0, 0.3, 0.1)) == 3
+    assert np.array_equal(mesh_cpu.time_grid(0.0, 0.3, 0.1), np.array([0.0, 0.15, 0.3]))
+    g = mesh_cpu.time_grid(-4000 * u.asec, 4000 * u.asec, 1 * u.asec)
+    assert len(g) == int(8000e-18 / 1e-18) + 1 and g[0] == -4000 * u.asec and g[-1] == 4000 * u.asec
+
+
+def test_data_indices_known_answers():
+    """sims.py:83-90: t0, every store_data_every-th index, and the last index; only the ends for < 1."""
+    assert list(mesh_cpu.data_indices(11, 4)) == [0, 4, 8, 10]
+    assert list(mesh_cpu.data_indices(12, 4)) == [0, 4, 8, 11]
+    assert list(mesh_cpu.data_indices(5, 1)) == [0, 1, 2, 3, 4]
+    assert list(mesh_cpu.data_indices(9, -1)) == [0, 8]
+    assert list(mesh_cpu.data_indices(9, 0)) == [0, 8]
+
+
 def test_arpack_and_lapack_eigenbases_agree():
     """The reference's eigsh (meshes.py:1303-1305) vs eigh_tridiagonal on one H_l."""
     o = mesh_cpu.SHOracle(r_bound=40 * u.bohr_radius, r_points=300, l_bound=3, potential=physics.coulomb,
