@@ -79,6 +79,22 @@ int tdse_tdma_batched(tdse_ctx *ctx, const double *sub, const double *diag, cons
                       double *x, const int64_t *offsets, const int64_t *lengths, int64_t n_systems);
 
 /*
+ * Allocation-free form of tdse_tdma_batched. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (tight serving loops,
+ * code that forbids hipMalloc). sub, diag, sup, d and x are DEVICE arrays of
+ * `total` complex128 each, owned by the caller (x is written); offsets and
+ * lengths are host arrays of n_systems, every system inside [0, total). The
+ * Thomas scratch (primed super-diagonal) and the staged offsets / lengths live
+ * in a static device array of the library, one per GPU, so the call allocates
+ * nothing: total <= 2^21 unknowns and n_systems <= 2^16, else TDSE_ERR_ARG.
+ * Synchronous: returns after x is written. Calls that overlap on one GPU
+ * (several host threads or contexts) race on that array.
+ */
+int tdse_tdma_batched_noalloc(tdse_ctx *ctx, const double *sub, const double *diag, const double *sup,
+                              const double *d, double *x, const int64_t *offsets, const int64_t *lengths,
+                              int64_t n_systems, int64_t total);
+
+/*
  * SphericalHarmonicMesh simulation, length gauge, SplitInteractionOperator
  * (ionization/mesh/meshes.py:985-1136, mesh_operators.py:815-1080,
  * evolution_methods.py:80-123). All arrays are in device position order:

@@ -100,6 +100,10 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.c_void_p] + [c_double_p] * 5 + [c_int64_p, c_int64_p, ctypes.c_int64],
     ),
+    "tdse_tdma_batched_noalloc": (
+        ctypes.c_int,
+        [ctypes.c_void_p] + [ctypes.c_void_p] * 5 + [c_int64_p, c_int64_p, ctypes.c_int64, ctypes.c_int64],
+    ),
     "tdse_sh_add": (
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.POINTER(ShDesc)] + [c_double_p] * 5 + [c_int32_p, c_int32_p],

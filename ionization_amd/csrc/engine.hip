@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -201,6 +202,7 @@ struct tdse_ctx {
     hipGraph_t graph_k = nullptr;  // kGraphSteps steps in one graph (fewer launches for small batches)
     hipGraphExec_t exec_k = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    double2* scratch = nullptr;  // this device's copy of `scratch` (tdse_tdma_batched_noalloc)
     double last_run_ms = 0.0;
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
@@ -766,6 +768,38 @@ __global__ void __launch_bounds__(256) k_stream_copy(const double2* __restrict__
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) st2(dst, i, ld2(src, i));
 }
 
+// Scratch of the allocation-free Thomas entry point: the primed super-diagonal
+// of up to kNoAllocPoints unknowns, then kNoAllocSystems int64 offsets and as
+// many int64 lengths (33 MiB in all). Each device holds its own copy until the
+// process ends. In host code the name `scratch` is only a host-side stand-in:
+// kernels and copies use the device address that hipGetSymbolAddress returns
+// for the current device (device_scratch), never the name itself.
+constexpr int64_t kNoAllocPoints = int64_t(1) << 21;
+constexpr int64_t kNoAllocSystems = int64_t(1) << 16;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double2 scratch[kNoAllocPoints + kNoAllocSystems];
+
+std::mutex g_scratch_mu;
+double2* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (which must be current), looked up
+// once per device. Returns nullptr with *err set on failure.
+double2* device_scratch(int device, hipError_t* err) {
+    *err = hipSuccess;
+    if (device < 0 || device >= kMaxDevices) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        double2* p = nullptr;
+        *err = hipGetSymbolAddress(reinterpret_cast<void**>(&p), HIP_SYMBOL(scratch));
+        if (*err != hipSuccess) return nullptr;
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+
 void free_blocks(tdse_ctx* ctx, std::vector<void*>& blocks) {
     for (void* p : blocks) ctx->arena.dealloc(p);
     blocks.clear();
@@ -823,6 +857,12 @@ int tdse_ctx_create(int device, size_t pool_bytes, tdse_ctx** out) {
     if (!ok) {
         tdse_ctx_destroy(ctx);
         return fail(nullptr, TDSE_ERR_HIP, "context allocation failed");
+    }
+    hipError_t se = hipSuccess;
+    ctx->scratch = device_scratch(device, &se);
+    if (!ctx->scratch) {
+        tdse_ctx_destroy(ctx);
+        return fail(nullptr, TDSE_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se));
     }
     *out = ctx;
     return TDSE_OK;
@@ -947,6 +987,44 @@ int tdse_tdma(tdse_ctx* ctx, const double* sub, const double* diag, const double
     if (n < 1) return fail(ctx, TDSE_ERR_ARG, "tdse_tdma: n must be >= 1");
     const int64_t off = 0;
     return tdse_tdma_batched(ctx, sub, diag, sup, d, x, &off, &n, 1);
+}
+
+int tdse_tdma_batched_noalloc(tdse_ctx* ctx, const double* sub, const double* diag, const double* sup,
+                              const double* d, double* x, const int64_t* offsets, const int64_t* lengths,
+                              int64_t n_systems, int64_t total) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    if (!sub || !diag || !sup || !d || !x || !offsets || !lengths || n_systems < 0 || total < 0)
+        return fail(ctx, TDSE_ERR_ARG, "tdse_tdma_batched_noalloc: NULL array or negative count");
+    if (total > kNoAllocPoints)
+        return fail(ctx, TDSE_ERR_ARG, "tdse_tdma_batched_noalloc: total " + std::to_string(total) +
+                                           " exceeds the scratch maximum of " + std::to_string(kNoAllocPoints) +
+                                           " unknowns");
+    if (n_systems > kNoAllocSystems)
+        return fail(ctx, TDSE_ERR_ARG, "tdse_tdma_batched_noalloc: " + std::to_string(n_systems) +
+                                           " systems exceed the scratch maximum of " +
+                                           std::to_string(kNoAllocSystems));
+    for (int64_t i = 0; i < n_systems; ++i)
+        if (lengths[i] < 1 || offsets[i] < 0 || offsets[i] > total - lengths[i])
+            return fail(ctx, TDSE_ERR_ARG, "tdse_tdma_batched_noalloc: system " + std::to_string(i) +
+                                               " lies outside [0, total)");
+    if (n_systems == 0) return TDSE_OK;
+    if (!ctx->scratch) return fail(ctx, TDSE_ERR_STATE, "tdse_tdma_batched_noalloc: no device scratch");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    double2* cp = ctx->scratch;
+    int64_t* doff = reinterpret_cast<int64_t*>(cp + kNoAllocPoints);
+    int64_t* dlen = doff + kNoAllocSystems;
+    const size_t nb = (size_t)n_systems * sizeof(int64_t);
+    HIP_TRY(ctx, hipMemcpyAsync(doff, offsets, nb, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(dlen, lengths, nb, hipMemcpyHostToDevice, ctx->stream));
+    const int threads = 64;
+    const int64_t grid = (n_systems + threads - 1) / threads;
+    hipLaunchKernelGGL(k_tdma_lines, dim3((unsigned)grid), dim3(threads), 0, ctx->stream,
+                       reinterpret_cast<const double2*>(sub), reinterpret_cast<const double2*>(diag),
+                       reinterpret_cast<const double2*>(sup), reinterpret_cast<const double2*>(d),
+                       reinterpret_cast<double2*>(x), cp, doff, dlen, n_systems);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return TDSE_OK;
 }
 
 int tdse_sh_add(tdse_ctx* ctx, const tdse_sh_desc* desc, const double* tables, const double* fs, const double* ts,
