@@ -82,8 +82,9 @@ class QuantumMesh:
         return np.abs(self.g) ** 2
 
     def evolve(self, time_step):
-        """One reference step on the device, then g refreshed (meshes.py:251-257)."""
-        self.sim._device_single_step(time_step)
+        """One step by the evolution method, then the mask (meshes.py:251-257)."""
+        self.g = self.spec.evolution_method.evolve(self, self.g, time_step)
+        self.g *= self.spec.mask(r=self.r_mesh)
 
 
 class LineMesh(QuantumMesh):
