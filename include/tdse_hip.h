@@ -271,6 +271,20 @@ int tdse_download_norm_by_l(tdse_ctx *ctx, int32_t slot, double *out);
  */
 int tdse_upload_state(tdse_ctx *ctx, int32_t kind, int32_t slot, const double *g, int32_t step);
 
+/*
+ * Health flags of simulations first_slot .. first_slot + n_slots - 1 of one
+ * kind (0 SH, 1 Line, 2 Cyl), one int32 each: a bitwise OR of TDSE_FLAG_*,
+ * set on the device where a data time's norm is stored and cleared by
+ * tdse_upload_state. TDSE_FLAG_NONFINITE: the norm was NaN or infinite (a
+ * diverged simulation); TDSE_FLAG_NORM_GROWTH: it exceeded the first or the
+ * previous stored norm by more than 0.1 %, the two conditions
+ * MeshSimulation.check warns about (ionization/mesh/sims.py:228-240). Slots
+ * that are not live read as whatever they last held. Waits for the context's
+ * stream.
+ */
+enum { TDSE_FLAG_NONFINITE = 1, TDSE_FLAG_NORM_GROWTH = 2 };
+int tdse_sim_flags(tdse_ctx *ctx, int32_t kind, int32_t first_slot, int32_t n_slots, int32_t *flags);
+
 /* Release a simulation's slot and device memory (serving retirement). */
 int tdse_remove(tdse_ctx *ctx, int32_t kind, int32_t slot);
 

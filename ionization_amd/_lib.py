@@ -11,6 +11,9 @@ import os
 
 from .exceptions import DeviceError
 
+FLAG_NONFINITE = 1     # TDSE_FLAG_NONFINITE: a stored norm was NaN or infinite
+FLAG_NORM_GROWTH = 2   # TDSE_FLAG_NORM_GROWTH: a stored norm grew by more than 0.1 %
+
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtdse_hip.so")
 
 c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -134,6 +137,7 @@ SIGNATURES = {
         [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, c_double_p, ctypes.c_int32],
     ),
     "tdse_remove": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "tdse_sim_flags": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, c_int32_p]),
     "tdse_materialize": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "tdse_last_run_ms": (ctypes.c_int, [ctypes.c_void_p, c_double_p]),
     "tdse_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),

@@ -74,6 +74,20 @@ __device__ __forceinline__ double2 crecip(double2 z) {
 }
 __device__ __forceinline__ double2 cdiv(double2 a, double2 b) { return cmul(a, crecip(b)); }
 
+// Health of a simulation, judged where its norm is stored at a data time:
+// TDSE_FLAG_NONFINITE for a norm that is NaN or infinite, TDSE_FLAG_NORM_GROWTH
+// for one more than 0.1 % above the first or the previous stored norm (the
+// checks of MeshSimulation.check, ionization/mesh/sims.py:228-240). Stored
+// norms that are zero (not recorded, e.g. before a resumed state) are skipped.
+enum { TDSE_HEALTH_NONFINITE = 1, TDSE_HEALTH_NORM_GROWTH = 2 };
+__device__ __forceinline__ int norm_health(const double* out_norm, int slot, double nrm) {
+    if (!(fabs(nrm) <= 1.7976931348623157e308)) return TDSE_HEALTH_NONFINITE;
+    if (slot == 0) return 0;
+    const double first = out_norm[0], prev = out_norm[slot - 1];
+    return ((first > 0.0 && nrm > 1.001 * first) || (prev > 0.0 && nrm > 1.001 * prev)) ? TDSE_HEALTH_NORM_GROWTH
+                                                                                          : 0;
+}
+
 // Sum over the 64 lanes of a wave; every lane gets the total.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -57,6 +57,7 @@ struct CylSim {
     int active;
     int phase;  // 1: g is the reference g after `step` steps; 0: the explicit rho half of step `step` is applied
     int hold;   // 1: end every step at an exact state (phase 1)
+    int flags;  // health flags (norm_health), device-updated at data times
 };
 
 // 1 / sqrt(x): the hardware estimate y refined by the series
@@ -367,8 +368,14 @@ __global__ void __launch_bounds__(256) k_cyl_observe(CylSim* __restrict__ sims, 
         __syncthreads();
     }
     if (t == 0) {
-        if (jb.y < 0) S.out_norm[slot] = red_re[0] * S.delta;
-        else S.out_ip[(size_t)slot * S.n_basis + jb.y] = cmk(red_re[0] * S.delta, red_im[0] * S.delta);
+        if (jb.y < 0) {
+            const double nrm = red_re[0] * S.delta;
+            const int health = norm_health(S.out_norm, slot, nrm);
+            S.out_norm[slot] = nrm;
+            if (health) S.flags |= health;
+        } else {
+            S.out_ip[(size_t)slot * S.n_basis + jb.y] = cmk(red_re[0] * S.delta, red_im[0] * S.delta);
+        }
     }
 }
 

@@ -32,6 +32,9 @@ constexpr int kMaxL = 4096;          // largest l_bound accepted (c_l table size
 constexpr int kSlotCapacity = 8192;  // simulations of each kind per context
 constexpr int kSideStreams = 3;      // extra streams for concurrent R-phase buckets (default priority)
 constexpr int kMaxPairThreads = 512; // largest workgroup of the two-line (velocity odd l-pair) R-phase
+constexpr int kNoFlags = 0;          // health flags of a freshly uploaded state
+static_assert(TDSE_FLAG_NONFINITE == TDSE_HEALTH_NONFINITE && TDSE_FLAG_NORM_GROWTH == TDSE_HEALTH_NORM_GROWTH,
+              "health flag values of the ABI and the kernels differ");
 
 thread_local std::string g_lib_error;
 
@@ -1508,6 +1511,8 @@ int tdse_upload_state(tdse_ctx* ctx, int32_t kind, int32_t slot, const double* g
                                hipMemcpyHostToDevice));
         HIP_TRY(ctx, hipMemcpy(reinterpret_cast<char*>(ctx->d_sh + slot) + offsetof(ShSim, phase), &st[1], sizeof(int),
                                hipMemcpyHostToDevice));
+        HIP_TRY(ctx, hipMemcpy(reinterpret_cast<char*>(ctx->d_sh + slot) + offsetof(ShSim, flags), &kNoFlags,
+                               sizeof(int), hipMemcpyHostToDevice));
     } else if (kind == 1) {
         if (slot < 0 || slot >= (int)ctx->line.size() || !ctx->line[slot].live)
             return fail(ctx, TDSE_ERR_ARG, "bad Line slot");
@@ -1516,6 +1521,8 @@ int tdse_upload_state(tdse_ctx* ctx, int32_t kind, int32_t slot, const double* g
         HIP_TRY(ctx, hipMemcpy(S.g, g, (size_t)S.npad * sizeof(double2), hipMemcpyHostToDevice));
         HIP_TRY(ctx, hipMemcpy(reinterpret_cast<char*>(ctx->d_line + slot) + offsetof(LineSim, step), &step, sizeof(int),
                                hipMemcpyHostToDevice));
+        HIP_TRY(ctx, hipMemcpy(reinterpret_cast<char*>(ctx->d_line + slot) + offsetof(LineSim, flags), &kNoFlags,
+                               sizeof(int), hipMemcpyHostToDevice));
     } else if (kind == 2) {
         if (slot < 0 || slot >= (int)ctx->cyl.size() || !ctx->cyl[slot].live) return fail(ctx, TDSE_ERR_ARG, "bad Cyl slot");
         const CylSim& S = ctx->cyl[slot].dev;
@@ -1525,6 +1532,7 @@ int tdse_upload_state(tdse_ctx* ctx, int32_t kind, int32_t slot, const double* g
         char* base = reinterpret_cast<char*>(ctx->d_cyl + slot);
         HIP_TRY(ctx, hipMemcpy(base + offsetof(CylSim, step), &st[0], sizeof(int), hipMemcpyHostToDevice));
         HIP_TRY(ctx, hipMemcpy(base + offsetof(CylSim, phase), &st[1], sizeof(int), hipMemcpyHostToDevice));
+        HIP_TRY(ctx, hipMemcpy(base + offsetof(CylSim, flags), &kNoFlags, sizeof(int), hipMemcpyHostToDevice));
         ctx->cyl_pending = true;
     } else {
         return fail(ctx, TDSE_ERR_ARG, "unknown simulation kind");
@@ -1559,6 +1567,32 @@ int tdse_remove(tdse_ctx* ctx, int32_t kind, int32_t slot) {
         return fail(ctx, TDSE_ERR_ARG, "unknown simulation kind");
     }
     ctx->dirty = true;
+    return TDSE_OK;
+}
+
+int tdse_sim_flags(tdse_ctx* ctx, int32_t kind, int32_t first_slot, int32_t n_slots, int32_t* flags) {
+    if (!ctx) return fail(nullptr, TDSE_ERR_ARG, "ctx is NULL");
+    if (!flags || first_slot < 0 || n_slots < 0) return fail(ctx, TDSE_ERR_ARG, "tdse_sim_flags: bad arguments");
+    size_t count, pitch, off;
+    const char* base;
+    if (kind == 0) {
+        count = ctx->sh.size(), pitch = sizeof(ShSim), off = offsetof(ShSim, flags);
+        base = reinterpret_cast<const char*>(ctx->d_sh);
+    } else if (kind == 1) {
+        count = ctx->line.size(), pitch = sizeof(LineSim), off = offsetof(LineSim, flags);
+        base = reinterpret_cast<const char*>(ctx->d_line);
+    } else if (kind == 2) {
+        count = ctx->cyl.size(), pitch = sizeof(CylSim), off = offsetof(CylSim, flags);
+        base = reinterpret_cast<const char*>(ctx->d_cyl);
+    } else {
+        return fail(ctx, TDSE_ERR_ARG, "unknown simulation kind");
+    }
+    if ((size_t)first_slot + (size_t)n_slots > count) return fail(ctx, TDSE_ERR_ARG, "tdse_sim_flags: slot range");
+    if (n_slots == 0) return TDSE_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemcpy2DAsync(flags, sizeof(int32_t), base + (size_t)first_slot * pitch + off, pitch,
+                                  sizeof(int32_t), (size_t)n_slots, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return TDSE_OK;
 }
 

@@ -40,6 +40,7 @@ struct LineSim {
     int method;            // LINE_ADI, LINE_SIO_LEN or LINE_SIO_VEL (real time steps only for the split operator)
     int step;
     int active;
+    int flags;             // health flags (norm_health), device-updated at data times
 };
 
 enum { LN_H = 0, LN_AP = 1, LN_AN = 2, LN_ZQ = 3, LN_MASK = 4, LN_NTAB = 5 };
@@ -135,13 +136,17 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 }
 
 template <int M>
-__device__ __forceinline__ void line_observe(const LineSim& S, const double2 (&x)[M], int k, int P, int slot,
+__device__ __forceinline__ void line_observe(LineSim& S, const double2 (&x)[M], int k, int P, int slot,
                                              double* red) {
     double acc = 0.0;
 #pragma unroll
     for (int i = 0; i < M; ++i) acc += cabs2(x[i]);
     const double nrm = block_sum(acc, red);
-    if (threadIdx.x == 0) S.out_norm[slot] = nrm * S.delta_z;
+    if (threadIdx.x == 0) {
+        const int health = norm_health(S.out_norm, slot, nrm * S.delta_z);
+        S.out_norm[slot] = nrm * S.delta_z;
+        if (health) S.flags |= health;
+    }
     for (int b = 0; b < S.n_basis; ++b) {
         const double2* phi = S.basis + (size_t)b * S.npad + k;
         double re = 0.0, im = 0.0;

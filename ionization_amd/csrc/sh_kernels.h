@@ -53,6 +53,7 @@ struct ShSim {
     int norm_by_l;         // 1: also reduce the norm of every l at data times
     int adi;               // 1: AlternatingDirectionImplicit (fs is then zero: the L-phase only masks and observes)
     const double* fa;      // ADI: [N] field times tau ratio per step, E(t_{n+1} + dt/2) tau_n / tau0
+    int flags;             // health flags (norm_health), device-updated at data times
 };
 
 constexpr int kShSegRows = 64;  // most rows of one L-phase wave (its l-segment)
@@ -356,8 +357,11 @@ __global__ void __launch_bounds__(256) k_sh_observe(ShSim* __restrict__ sims, co
     }
     if (t == 0) {
         if (jb.y < 0) {
-            S.out_norm[slot] = red_re[0] * S.delta_r;
+            const double nrm = red_re[0] * S.delta_r;
+            const int health = norm_health(S.out_norm, slot, nrm);
+            S.out_norm[slot] = nrm;
             S.out_lnorm[slot] = red_im[0] * S.delta_r;
+            if (health) S.flags |= health;
         } else {
             S.out_ip[(size_t)slot * S.n_basis + jb.y] = cmk(red_re[0] * S.delta_r, red_im[0] * S.delta_r);
         }

@@ -30,6 +30,8 @@ from .exceptions import DeviceError
 KIND_SH = 0
 KIND_LINE = 1
 KIND_CYL = 2
+FLAG_NONFINITE = _lib.FLAG_NONFINITE
+FLAG_NORM_GROWTH = _lib.FLAG_NORM_GROWTH
 
 SH_NTAB = 6
 SH_RQ = 4  # row of the interaction's radial factor
@@ -790,6 +792,15 @@ class DeviceContext:
     def materialize(self, kind: int, slot: int):
         """Make the device g equal to the reference g after the current step count."""
         self._check(self._lib.tdse_materialize(self._ctx, kind, slot))
+
+    def flags(self, kind: int, first_slot: int = 0, n_slots: Optional[int] = None) -> np.ndarray:
+        """Health flags (FLAG_NONFINITE | FLAG_NORM_GROWTH) of a range of slots of one kind."""
+        if n_slots is None:
+            n_slots = 1
+        out = np.zeros(int(n_slots), dtype=np.int32)
+        self._check(self._lib.tdse_sim_flags(self._ctx, kind, int(first_slot), int(n_slots),
+                                             out.ctypes.data_as(_lib.c_int32_p)))
+        return out
 
     def remove(self, kind: int, slot: int):
         self._check(self._lib.tdse_remove(self._ctx, kind, slot))

@@ -63,6 +63,9 @@ class _DeviceSession:
             return self.ctx.download_cyl(self.slot)[0]
         return self.ctx.download_line(self.slot)[0]
 
+    def flags(self):
+        return int(self.ctx.flags(self.kind, self.slot)[0])
+
     def data(self):
         if self.kind == engine.KIND_SH:
             res = self.ctx.download_sh(self.slot, with_g=False)
@@ -159,29 +162,53 @@ class MeshSimulation:
         if dev["lnorms"] is not None:
             self.lnorm[slots] = dev["lnorms"][slots]
 
-    def _device_single_step(self, time_step):
-        """One reference step at the current time index (QuantumMesh.evolve's seam)."""
-        sess = self._session_open()
-        n = self.time_index - 1 if self.time_index > 0 else 0
-        sess.hold(True)
-        sess.ctx.upload_state(sess.kind, sess.slot, self.mesh.g, n)
-        sess.ctx.run(1, use_graph=False)
-        self.mesh.g = sess.g()
+    def _store_host_data(self):
+        """The datastores' values at the current data index from the host copy of g (sims.py:222-226)."""
+        for ds in self.datastores_by_type.values():
+            ds.store_host()
+        if self.mesh.g.ndim == 2 and isinstance(self.mesh, meshes.SphericalHarmonicMesh):
+            self.lnorm[self.data_time_index] = self.mesh.norm(self.mesh.g[-1])
+
+    def _check_data_times(self, first_time_index=0):
+        """check() at every stored data time from first_time_index on, with time_index and data_time_index
+        set as the reference's loop has them when it calls check (sims.py:290-294)."""
+        time_index, data_time_index = self.time_index, self.data_time_index
+        try:
+            for i in np.nonzero(self.data_indices >= first_time_index)[0]:
+                self.data_time_index = int(i)
+                self.time_index = int(self.data_indices[i])
+                self.check()
+        finally:
+            self.time_index, self.data_time_index = time_index, data_time_index
+
+    def _check_device_health(self, flags):
+        """Record the device's health flags (a non-finite norm: the simulation diverged)."""
+        if flags & engine.FLAG_NONFINITE and not self.warnings.get("non_finite"):
+            msg = f"Wavefunction norm is not finite for {self}: the evolution diverged"
+            logger.error(msg)
+            self.warnings["non_finite"].append(WarningRecord(int(self.time_index), msg))
 
     # ---------------------------------------------------------- run loop
 
     def check(self):
-        """Norm-growth warnings at every stored data time (sims.py:228-240)."""
-        norm = self.data.norm if data.Norm in self.datastores_by_type else None
-        if norm is None:
+        """Norm-growth warnings at the current data index (sims.py:228-240): above the initial norm or
+        the previous stored norm by more than 0.1 %. At data index 0 the previous norm is the last,
+        still blank (NaN) entry, which compares false, as in the reference."""
+        if data.Norm not in self.datastores_by_type:
             return
-        for i in range(1, self.data_time_steps):
-            if np.isnan(norm[i]):
-                continue
-            if norm[i] > 1.001 * norm[0]:
-                logger.warning(f"Wavefunction norm ({norm[i]}) has exceeded initial norm ({norm[0]}) by more than .1% for {self}")
-            if norm[i] > 1.001 * norm[i - 1]:
-                logger.warning(f"Wavefunction norm ({norm[i]}) at data index {i} has exceeded the previous norm by more than .1% for {self}")
+        norms = self.data.norm
+        i = self.data_time_index
+        norm = norms[i]
+        prev = norms[i - 1] if i > 0 else np.nan  # the reference reads norms[-1], still blank at index 0
+        if norm > 1.001 * norms[0]:
+            msg = f"Wavefunction norm ({norm}) has exceeded initial norm ({norms[0]}) by more than .1% for {self}"
+            logger.warning(msg)
+            self.warnings["norm_growth"].append(WarningRecord(int(self.time_index), msg))
+        if norm > 1.001 * prev:
+            msg = (f"Wavefunction norm ({norm}) at time_index = {i} has exceeded norm from previous time step "
+                   f"({prev}) by more than .1% for {self}")
+            logger.warning(msg)
+            self.warnings["norm_growth"].append(WarningRecord(int(self.time_index), msg))
 
     def run(self, progress_bar: bool = False, callback: Optional[Callable] = None,
             checkpoint_callback: Optional[Callable] = None, chunk_steps: int = 2000):
@@ -191,17 +218,15 @@ class MeshSimulation:
         self.status = "running"
         try:
             if callback is not None:
-                # per-step: the reference exposes g after every step to the callback
-                sess.hold(True)
-                while True:
-                    callback(self)
-                    if self.time_index == self.time_steps - 1:
-                        break
-                    sess.ctx.run(1)
-                    self.time_index += 1
-                    self.mesh.g = sess.g()
-                sess.ctx.run(1)  # final data point
+                self._run_with_callback(sess, callback)
             else:
+                first_collected = start_index
+                if start_index > 0 and sess.kind != engine.KIND_SH and start_index in self.data_indices:
+                    # Line and cylinder kernels observe after each step: the resumed state's own data
+                    # time is stored from the host copy, as the reference's loop does (sims.py:290-294)
+                    self.data_time_index = int(np.searchsorted(self.data_indices, start_index))
+                    self._store_host_data()
+                    first_collected = start_index + 1
                 total = self.time_steps  # N + 1 replays from step 0
                 done = sess.step_index()
                 while done < total:
@@ -215,21 +240,42 @@ class MeshSimulation:
                     else:
                         sess.ctx.run(k)
                     done += k
+                    self._check_device_health(sess.flags())
                     if self.spec.checkpoints and done < total:
                         now = datetime.datetime.utcnow()
                         if (now - self.latest_checkpoint_time) > self.spec.checkpoint_every:
                             self.time_index = sess.step_index()
                             self.mesh.g = sess.g()
-                            self._collect(start_index)
+                            self._collect(first_collected)
                             self.do_checkpoint(now, checkpoint_callback)
+                self.mesh.g = sess.g()
+                self._collect(first_collected)
+                self._check_data_times(start_index)
             self.time_index = self.time_steps - 1
             self.data_time_index = self.data_time_steps
-            self.mesh.g = sess.g()
-            self._collect(start_index)
-            self.check()
             self.status = "finished"
         finally:
             self._session_close()
+
+    def _run_with_callback(self, sess, callback):
+        """The reference's per-step loop (sims.py:289-321): at every data time the datastores are stored
+        from g and check() runs before the callback; then one device step, g materialised."""
+        sess.hold(True)
+        self.data_time_index = int(np.searchsorted(self.data_indices, self.time_index))
+        while True:
+            is_data_time = self.time_index in self.data_indices
+            if is_data_time:
+                self._store_host_data()
+                self.check()
+            callback(self)
+            if is_data_time:
+                self.data_time_index += 1
+            if self.time_index == self.time_steps - 1:
+                break
+            sess.ctx.run(1)
+            self.time_index += 1
+            self.mesh.g = sess.g()
+        self._check_device_health(sess.flags())
 
     def do_checkpoint(self, now, callback=None):
         """Pickle the simulation with its mesh (sims.py:348-356)."""
@@ -271,17 +317,20 @@ class SphericalHarmonicSimulation(MeshSimulation):
     """SH simulation with the largest-l norm check (sims.py:973-990)."""
 
     def check(self):
+        """check() at the current data index plus the largest-l warning (sims.py:979-990); the norm in
+        the largest l at that data time is reduced on the device (or taken from g per step)."""
         super().check()
-        for i in range(self.data_time_steps):
-            if np.isnan(self.lnorm[i]):
-                continue
-            # state_overlap(g_L, g_L) = |<g_L|g_L>|^2 in the reference
-            norm_in_largest_l = self.lnorm[i] ** 2
-            if norm_in_largest_l > 1e-6:
-                msg = (f"Wavefunction norm in largest angular momentum state is large at time index "
-                       f"{self.data_indices[i]} (norm at bound = {norm_in_largest_l}), consider increasing l bound")
-                logger.warning(msg)
-                self.warnings["norm_in_largest_l"].append(WarningRecord(int(self.data_indices[i]), msg))
+        lnorm = self.lnorm[self.data_time_index]
+        # state_overlap(g_L, g_L) = |<g_L|g_L>|^2 in the reference
+        norm_in_largest_l = lnorm ** 2
+        if norm_in_largest_l > 1e-6:
+            fraction = norm_in_largest_l / self.data.norm[self.data_time_index] if data.Norm in \
+                self.datastores_by_type else float("nan")
+            msg = (f"Wavefunction norm in largest angular momentum state is large at time index {self.time_index} "
+                   f"(norm at bound = {norm_in_largest_l}, fraction of norm = {fraction}), "
+                   f"consider increasing l bound")
+            logger.warning(msg)
+            self.warnings["norm_in_largest_l"].append(WarningRecord(int(self.time_index), msg))
 
 
 class MeshSpecification:

@@ -15,6 +15,9 @@ simulations share the device (every kernel works per simulation in a fixed
 order), so a served result equals the result of running it alone.
 
 Time-to-result is measured from arrival to the observables being on the host.
+After every chunk the device health flags of the resident simulations are
+read; one whose norm became non-finite (it diverged) is retired at once with
+``Request.error`` set, and the others carry on.
 """
 
 import concurrent.futures
@@ -48,6 +51,7 @@ class Request:
     g: Optional[np.ndarray] = None
     norms: Optional[np.ndarray] = None
     ips: Optional[np.ndarray] = None
+    error: Optional[str] = None  # set when the simulation was retired early (it diverged)
 
     @property
     def latency(self):
@@ -134,8 +138,32 @@ class Server:
                engine.KIND_CYL: self.ctx.add_cyl}[r.sim_kind]
         r.slot = add(c)
 
-    def _retire(self, r, now):
+    def _diverged(self, resident):
+        """Resident requests whose device health flags report a non-finite norm (one read per kind)."""
+        out = []
+        for kind in (engine.KIND_SH, engine.KIND_LINE, engine.KIND_CYL):
+            rs = [r for r in resident if r.sim_kind == kind]
+            if not rs:
+                continue
+            flags = self.ctx.flags(kind, 0, max(r.slot for r in rs) + 1)
+            out += [r for r in rs if flags[r.slot] & engine.FLAG_NONFINITE]
+        return out
+
+    def _retire(self, r, now, error=None):
         ctx = self.ctx
+        r.error = error
+        if error is not None:
+            # a diverged simulation is stopped where it is: its stored data and state go back as they are
+            if r.sim_kind == engine.KIND_SH:
+                res = ctx.download_sh(r.slot, with_g=self.keep_g)
+                r.g, r.norms, r.ips = res.g, res.norms, res.ips
+            elif r.sim_kind == engine.KIND_CYL:
+                r.g, r.norms, r.ips = ctx.download_cyl(r.slot, with_g=self.keep_g)
+            else:
+                r.g, r.norms, r.ips = ctx.download_line(r.slot)
+            ctx.remove(r.sim_kind, r.slot)
+            r.t_done = now()
+            return
         ctx.materialize(r.sim_kind, r.slot)
         if r.sim_kind == engine.KIND_SH:
             res = ctx.download_sh(r.slot, with_g=self.keep_g)
@@ -163,6 +191,7 @@ class Server:
         reserved = 0
         replays = 0
         rebuilds = 0
+        retired = False
         done = 0
         while done < len(reqs):
             t = now()
@@ -202,20 +231,28 @@ class Server:
                     raise DeviceError(f"request {waiting[0].rid} does not fit the device pool")
                 time.sleep(0.001)
                 continue
-            if changed:
-                rebuilds += 1
+            if changed or retired:
+                rebuilds += 1  # the resident set changed: the next replay captures a new step graph
             # run until the next simulation finishes or the chunk ends
             left = min(r.admitted_replay + r.replays_to_finish() - replays for r in resident)
             k = max(1, min(self.chunk_steps, left))
             self.ctx.run(k)
             replays += k
-            # retire finished simulations
+            # retire finished simulations, and diverged ones with an error (the others go on)
+            retired = False
             still = []
+            diverged = {id(r) for r in self._diverged(resident)}
             for r in resident:
-                if replays - r.admitted_replay >= r.replays_to_finish():
+                if id(r) in diverged:
+                    self._retire(r, now, error="non-finite norm: the simulation diverged")
+                    reserved -= self._reserve(r.compiled)
+                    done += 1
+                    retired = True
+                elif replays - r.admitted_replay >= r.replays_to_finish():
                     self._retire(r, now)
                     reserved -= self._reserve(r.compiled)
                     done += 1
+                    retired = True
                 else:
                     still.append(r)
             resident = still
@@ -234,12 +271,14 @@ def _line_ground_state(z_points=8192):
 
 
 def served_mix(n: int, rate_per_s: float, seed: int = 0, *, single_steps: Optional[int] = None,
+               scan_steps: Optional[int] = None, line_steps: Optional[int] = None,
                line_ground_state=None) -> List[Request]:
     """
     A seeded request stream: exponential inter-arrival times at `rate_per_s`,
     each request drawn as 70 % sh_single-like (200 l x 1000 r with a drawn
     fluence and phase), 20 % an sh_scan draw and 10 % line_cn (8192 x 4000).
-    `single_steps` shortens the sh_single-like runs (tests); the Line ground
+    `single_steps`, `scan_steps` and `line_steps` shorten the runs of each
+    class to that many steps at the same mesh sizes (tests); the Line ground
     state is computed once and shared.
     """
     from . import workloads
@@ -273,14 +312,17 @@ def served_mix(n: int, rate_per_s: float, seed: int = 0, *, single_steps: Option
             p = scan[i]
 
             def make(p=p):
-                sim = workloads.sh_scan_spec(p).to_sim()
+                spec = workloads.sh_scan_spec(p)
+                if scan_steps is not None:
+                    workloads.fixed_step_count(spec, scan_steps)
+                sim = spec.to_sim()
                 return sim.spec.evolution_method.compile(sim)
 
             out.append(Request(i, "sh_scan", t, make))
         else:
             def make(i=i):
-                spec = workloads.line_cn_spec(f"line_{i}", z_points=8192, n_steps=4000, store_data_every=100,
-                                              ground_state=line_ground_state)
+                spec = workloads.line_cn_spec(f"line_{i}", z_points=8192, n_steps=line_steps or 4000,
+                                              store_data_every=100, ground_state=line_ground_state)
                 sim = spec.to_sim()
                 return spec.evolution_method.compile(sim)
 

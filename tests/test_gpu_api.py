@@ -214,3 +214,60 @@ def test_checkpoint_resume_other_sh_methods(tmp_path, method):
     resumed.run()
     assert rel_l2(resumed.mesh.g, full.mesh.g) < 1e-13
     assert np.allclose(resumed.data.norm, full.data.norm, rtol=0, atol=1e-13)
+
+
+def test_callback_sees_the_current_data_time_stored_and_checked():
+    """Callback mode follows sims.py:289-313: at a data time the datastores are stored from g and
+    check() has run before the callback, which sees sim.data at sim.data_time_index."""
+    seen = []
+
+    def cb(s):
+        if s.time_index in s.data_indices:
+            i = s.data_time_index
+            seen.append((s.time_index, int(s.data_indices[i]), float(s.data.norm[i]), s.lnorm[i]))
+
+    a = _spec("cbd", time_final=-40 * u.asec, store_data_every=4).to_sim()
+    a.run(callback=cb)
+    assert [t for t, *_ in seen] == list(a.data_indices)
+    assert all(t == di and np.isfinite(n) and np.isfinite(ln) for t, di, n, ln in seen)
+    b = _spec("cbd2", time_final=-40 * u.asec, store_data_every=4).to_sim()
+    b.run()
+    assert np.allclose(a.data.norm, b.data.norm, rtol=0, atol=1e-13)
+    assert np.allclose(a.lnorm, b.lnorm, rtol=0, atol=1e-13)
+
+
+def _resume_case(kind):
+    if kind == "line":
+        full = workloads.line_cn_spec("full", z_points=1024, n_steps=120, store_data_every=1).to_sim()
+        gs = full.spec.initial_state.g
+        mk = lambda: workloads.line_cn_spec("part", z_points=1024, n_steps=120, store_data_every=1,
+                                            ground_state=gs).to_sim()
+        return full, mk
+    mk = lambda: _cyl_spec("part", store_data_every=1).to_sim()
+    return _cyl_spec("full", store_data_every=1).to_sim(), mk
+
+
+@pytest.mark.parametrize("kind", ["line", "cyl"])
+def test_line_and_cyl_checkpoint_resume_data_equal_uninterrupted(tmp_path, kind):
+    """A resumed Line or cylinder run keeps the data at the checkpoint's own index (the device
+    observes after each step; the resumed state's data time is stored from g)."""
+    full, mk = _resume_case(kind)
+    full.run()
+    part = mk()
+    sess = part._session_open()
+    sess.hold(True)
+    sess.ctx.run(40)
+    part.time_index = sess.step_index()
+    assert part.time_index == 40
+    part.mesh.g = sess.g()
+    part._collect(0)
+    path = part.save(str(tmp_path))
+    part._session_close()
+    resumed = mesh.MeshSimulation.load(path)
+    resumed.run()
+    assert rel_l2(resumed.mesh.g, full.mesh.g) < 1e-13
+    assert np.all(np.isfinite(resumed.data.norm))
+    assert np.allclose(resumed.data.norm, full.data.norm, rtol=0, atol=1e-13)
+    for st in full.spec.test_states:
+        assert np.allclose(resumed.data.inner_products[st], full.data.inner_products[st], rtol=0, atol=1e-13)
+    assert not resumed.warnings.get("norm_growth")

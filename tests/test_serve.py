@@ -47,6 +47,9 @@ class FakeContext:
         self.log.append(("remove", slot))
         del self.slots[slot]
 
+    def flags(self, kind, first, n):
+        return np.zeros(n, np.int32)
+
     def close(self):
         pass
 
@@ -106,3 +109,36 @@ def test_served_mix_is_seeded_and_mixes_the_three_classes():
     assert abs(frac["line_cn"] - 0.1) < 0.06
     gaps = np.diff([0.0] + [r.arrival for r in a])
     assert np.all(gaps > 0) and abs(np.mean(gaps) - 0.1) < 0.03
+
+
+class FlaggingContext(FakeContext):
+    """A stand-in whose device reports a non-finite norm for chosen slots after a given replay."""
+
+    def __init__(self, bad_after):
+        super().__init__()
+        self.bad_after = bad_after  # slot -> replays after which its norm is non-finite
+
+    def flags(self, kind, first, n):
+        out = np.zeros(n, np.int32)
+        for s, v in self.slots.items():
+            if first <= s < first + n and s in self.bad_after and v["replays"] >= self.bad_after[s]:
+                out[s - first] = engine.FLAG_NONFINITE
+        return out
+
+    def materialize(self, kind, slot):
+        assert slot not in self.bad_after, "a diverged simulation is not materialised"
+        super().materialize(kind, slot)
+
+
+def test_diverged_simulation_is_retired_with_an_error_and_the_others_finish():
+    ctx = FlaggingContext({1: 8})
+    with serve.Server(pool_bytes=1 << 30, ctx=ctx, chunk_steps=5) as srv:
+        rep = srv.run(_requests([20, 40, 12]), wait_compiled=True)
+    by_id = {r.rid: r for r in rep.requests}
+    assert by_id[1].error is not None and "diverged" in by_id[1].error
+    assert by_id[0].error is None and by_id[2].error is None
+    assert not ctx.slots
+    # the diverged one left after the first chunk that saw its flag, long before its 41 replays
+    removes = [i for i, e in enumerate(ctx.log) if e == ("remove", 1)]
+    runs_before = sum(e[1] for e in ctx.log[: removes[0]] if e[0] == "run")
+    assert runs_before == 10 and rep.replays == 21
