@@ -14,6 +14,12 @@ from .. import exceptions
 from ..summables import Summand, Sum
 
 
+def _require(ok, error_type, owner, message):
+    """Raise ``error_type`` naming the class whose parameter failed (the reference's error types)."""
+    if not ok:
+        raise error_type(f"{owner}: {message}")
+
+
 # ---------------------------------------------------------------- potentials
 
 
@@ -88,21 +94,13 @@ class HarmonicOscillator(PotentialEnergy):
 class SoftCoulombPotential(PotentialEnergy):
     """k q Q / sqrt(r^2 + s^2) (``potentials/static.py:54-91``)."""
 
-    def __init__(
-        self,
-        charge: float = 1 * u.proton_charge,
-        softening_distance: float = 0.05 * u.bohr_radius,
-    ):
+    def __init__(self, charge: float = 1 * u.proton_charge, softening_distance: float = 0.05 * u.bohr_radius):
         self.charge = charge
         self.softening_distance = softening_distance
 
     def __call__(self, *, r, test_charge, **kwargs):
-        return (
-            u.coulomb_constant
-            * self.charge
-            * test_charge
-            / np.sqrt((r ** 2) + (self.softening_distance ** 2))
-        )
+        numerator = u.coulomb_constant * self.charge * test_charge
+        return numerator / np.sqrt(r ** 2 + self.softening_distance ** 2)
 
 
 # ---------------------------------------------------------------- windows
@@ -159,21 +157,12 @@ class SmoothedTrapezoidalWindow(TimeWindow):
         self.time_plateau = time_plateau
 
     def __call__(self, t):
-        cond_before = np.less(t, self.time_front)
-        cond_middle = np.less_equal(self.time_front, t) * np.less_equal(
-            t, self.time_front + self.time_plateau
-        )
-        cond_after = np.less(self.time_front + self.time_plateau, t) * np.less_equal(
-            t, (2 * self.time_front) + self.time_plateau
-        )
-        out = np.where(cond_before, np.sin(u.pi * t / (2 * self.time_front)) ** 2, 0)
-        out += np.where(cond_middle, 1, 0)
-        out += np.where(
-            cond_after,
-            np.cos(u.pi * (t - (self.time_front + self.time_plateau)) / (2 * self.time_front)) ** 2,
-            0,
-        )
-        return out
+        t = np.asarray(t, dtype=np.float64)
+        front, flat_end = self.time_front, self.time_front + self.time_plateau
+        # the rise has no lower bound: times before 0 follow sin^2 as well (windows.py:203-225)
+        rise = np.sin(0.5 * u.pi * t / front) ** 2
+        fall = np.cos(0.5 * u.pi * (t - flat_end) / front) ** 2
+        return np.select([t < front, t <= flat_end, t <= flat_end + front], [rise, 1.0, fall], default=0.0)
 
 
 # ---------------------------------------------------------------- fields
@@ -232,13 +221,44 @@ class UniformLinearlyPolarizedElectricPotential(ElectricPotential):
     def __call__(self, *, t, z, test_charge, **kwargs):
         return -z * test_charge * self.get_electric_field_amplitude(t)
 
-    def get_vector_potential_amplitude_numeric(self, times):
-        """-integral of E over ``times`` (pulses.py:73-78), by the rule of ``simps_cumulative``."""
-        return -simps_cumulative(self.get_electric_field_amplitude(times), times)[-1]
+    def get_electric_field_integral_numeric(self, times, rule: str = "simps"):
+        """Integral of E over ``times`` (pulses.py:59-72): rule ``'simps'`` is Simpson's rule as
+        ``scipy.integrate.simps`` computed it before SciPy 1.11 (``simps_cumulative``), ``'trapz'``
+        the trapezoid rule."""
+        times = np.asarray(times, dtype=np.float64)
+        e = np.asarray(self.get_electric_field_amplitude(times), dtype=np.float64) * np.ones(len(times))
+        if rule == "simps":
+            return simps_cumulative(e, times)[-1]
+        if rule == "trapz":
+            return float(np.sum(0.5 * (e[1:] + e[:-1]) * np.diff(times)))
+        raise ValueError(f"unknown integration rule {rule!r} (expected 'simps' or 'trapz')")
+
+    def get_vector_potential_amplitude_numeric(self, times, rule: str = "simps"):
+        """The vector potential at ``times[-1]``: minus the integral of E (pulses.py:73-77)."""
+        return -self.get_electric_field_integral_numeric(times, rule=rule)
+
+    def get_electric_field_integral_numeric_cumulative(self, times):
+        """Integral of E from ``times[0]`` to every time of ``times`` by the trapezoid rule, 0 first
+        (``cumtrapz(..., initial=0)``, pulses.py:79-83)."""
+        times = np.asarray(times, dtype=np.float64)
+        e = np.asarray(self.get_electric_field_amplitude(times), dtype=np.float64) * np.ones(len(times))
+        out = np.zeros(len(times))
+        out[1:] = np.cumsum(0.5 * (e[1:] + e[:-1]) * np.diff(times))
+        return out
 
     def get_vector_potential_amplitude_numeric_cumulative(self, times):
-        """The vector potential at every prefix of ``times``, by the same rule."""
-        return -simps_cumulative(self.get_electric_field_amplitude(times), times)
+        """Minus the cumulative trapezoid integral of E (pulses.py:84-87)."""
+        return -self.get_electric_field_integral_numeric_cumulative(times)
+
+    def get_fluence_numeric(self, times, rule: str = "simps"):
+        """eps0 c times the integral of E^2 over ``times`` (pulses.py:89-98)."""
+        times = np.asarray(times, dtype=np.float64)
+        e2 = np.abs(np.asarray(self.get_electric_field_amplitude(times), dtype=np.float64)) ** 2 * np.ones(len(times))
+        if rule == "simps":
+            return u.epsilon_0 * u.c * simps_cumulative(e2, times)[-1]
+        if rule == "trapz":
+            return u.epsilon_0 * u.c * float(np.sum(0.5 * (e2[1:] + e2[:-1]) * np.diff(times)))
+        raise ValueError(f"unknown integration rule {rule!r} (expected 'simps' or 'trapz')")
 
 
 class NoElectricPotential(UniformLinearlyPolarizedElectricPotential):
@@ -300,12 +320,10 @@ class SincPulse(UniformLinearlyPolarizedElectricPotential):
         omega_min: float = DEFAULT_OMEGA_MIN,
         **kwargs,
     ):
-        if pulse_width <= 0:
-            raise exceptions.InvalidPotentialParameter("pulse width must be positive")
-        if fluence < 0:
-            raise exceptions.InvalidPotentialParameter("fluence must be non-negative")
-        if omega_min <= 0:
-            raise exceptions.InvalidPotentialParameter("omega_min must be positive")
+        bad = exceptions.InvalidPotentialParameter
+        _require(pulse_width > 0, bad, "SincPulse", f"pulse_width must be > 0, got {pulse_width}")
+        _require(fluence >= 0, bad, "SincPulse", f"fluence must be >= 0, got {fluence}")
+        _require(omega_min > 0, bad, "SincPulse", f"omega_min must be > 0, got {omega_min}")
         super().__init__(**kwargs)
         self.omega_min = omega_min
         self.pulse_width = pulse_width
@@ -348,12 +366,10 @@ class GaussianPulse(UniformLinearlyPolarizedElectricPotential):
         pulse_center=DEFAULT_PULSE_CENTER,
         **kwargs,
     ):
-        if pulse_width <= 0:
-            raise exceptions.InvalidPotentialParameter("pulse width must be positive")
-        if fluence < 0:
-            raise exceptions.InvalidPotentialParameter("fluence must be non-negative")
-        if omega_carrier < 0:
-            raise exceptions.InvalidPotentialParameter("omega_carrier must be non-negative")
+        bad = exceptions.InvalidPotentialParameter
+        _require(pulse_width > 0, bad, "GaussianPulse", f"pulse_width must be > 0, got {pulse_width}")
+        _require(fluence >= 0, bad, "GaussianPulse", f"fluence must be >= 0, got {fluence}")
+        _require(omega_carrier >= 0, bad, "GaussianPulse", f"omega_carrier must be >= 0, got {omega_carrier}")
         super().__init__(**kwargs)
         self.omega_carrier = omega_carrier
         self.pulse_width = pulse_width
@@ -377,16 +393,11 @@ class GaussianPulse(UniformLinearlyPolarizedElectricPotential):
         pulse_center=DEFAULT_PULSE_CENTER,
         **kwargs,
     ):
-        """Carrier taken from a SincPulse of the same width and omega_min (``pulses.py:1064-1100``)."""
-        dummy = SincPulse(pulse_width=pulse_width, omega_min=omega_min)
-        pulse = cls(
-            pulse_width=pulse_width,
-            omega_carrier=dummy.omega_carrier,
-            fluence=fluence,
-            phase=phase,
-            pulse_center=pulse_center,
-            **kwargs,
-        )
+        """The carrier of a SincPulse with the same width and omega_min, w_min + pi / pulse_width
+        (``pulses.py:1064-1100``)."""
+        carrier = (omega_min + (omega_min + u.twopi / pulse_width)) / 2  # the sinc band's centre, as computed there
+        pulse = cls(pulse_width=pulse_width, omega_carrier=carrier, fluence=fluence, phase=phase,
+                    pulse_center=pulse_center, **kwargs)
         pulse.omega_min = omega_min
         return pulse
 
@@ -402,19 +413,14 @@ class GaussianPulse(UniformLinearlyPolarizedElectricPotential):
     ):
         """Fluence derived from the amplitude, then the amplitude pinned (``pulses.py:1151-1175``)."""
         fluence = cls.fluence_from_pulse_width_and_amplitude(pulse_width, amplitude)
-        pot = cls.from_omega_min(
-            pulse_width=pulse_width,
-            omega_min=omega_min,
-            fluence=fluence,
-            phase=phase,
-            pulse_center=pulse_center,
-            **kwargs,
-        )
-        if np.isclose(amplitude / pot.amplitude, 1):
-            pot.amplitude = amplitude
-        else:
-            raise ValueError("Given amplitude not close enough to calculated amplitude")
-        return pot
+        pulse = cls.from_omega_min(pulse_width=pulse_width, omega_min=omega_min, fluence=fluence, phase=phase,
+                                   pulse_center=pulse_center, **kwargs)
+        # the amplitude recomputed from the fluence agrees to rounding; keep the caller's exact value
+        if not np.isclose(amplitude / pulse.amplitude, 1):
+            raise ValueError(f"GaussianPulse.from_amplitude: amplitude {amplitude} does not round-trip through "
+                             f"the fluence (got {pulse.amplitude})")
+        pulse.amplitude = amplitude
+        return pulse
 
     def get_electric_field_envelope(self, t):
         tau = np.array(t) - self.pulse_center
@@ -459,25 +465,22 @@ class RadialCosineMask(Mask):
         outer_radius: float = 100 * u.bohr_radius,
         smoothness: float = 8,
     ):
-        if inner_radius < 0 or outer_radius < 0:
-            raise exceptions.InvalidMaskParameter("inner and outer radius must be non-negative")
-        if inner_radius >= outer_radius:
-            raise exceptions.InvalidMaskParameter("outer radius must be larger than inner radius")
-        if smoothness < 1:
-            raise exceptions.InvalidMaskParameter("smoothness must be greater than 1")
+        bad = exceptions.InvalidMaskParameter
+        _require(min(inner_radius, outer_radius) >= 0, bad, "RadialCosineMask",
+                 f"radii must be >= 0, got inner {inner_radius} and outer {outer_radius}")
+        _require(outer_radius > inner_radius, bad, "RadialCosineMask",
+                 f"outer_radius {outer_radius} must exceed inner_radius {inner_radius}")
+        _require(smoothness >= 1, bad, "RadialCosineMask", f"smoothness must be >= 1, got {smoothness}")
         self.inner_radius = inner_radius
         self.outer_radius = outer_radius
         self.smoothness = smoothness
 
     def __call__(self, *, r, **kwargs):
-        return np.where(
-            np.greater_equal(r, self.inner_radius) * np.less(r, self.outer_radius),
-            np.abs(
-                np.cos(0.5 * u.pi * (r - self.inner_radius) / np.abs(self.outer_radius - self.inner_radius))
-            )
-            ** (1 / self.smoothness),
-            np.where(np.greater_equal(r, self.outer_radius), 0, 1),
-        )
+        r = np.asarray(r, dtype=np.float64)
+        width = abs(self.outer_radius - self.inner_radius)
+        with np.errstate(invalid="ignore"):
+            ramp = np.abs(np.cos(0.5 * u.pi * (r - self.inner_radius) / width)) ** (1 / self.smoothness)
+        return np.select([r < self.inner_radius, r < self.outer_radius], [1.0, ramp], default=0.0)
 
     def __repr__(self):
         return (

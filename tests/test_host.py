@@ -373,3 +373,24 @@ def test_unsupported_method_combinations_raise():
                         evolution_method=mesh.AlternatingDirectionImplicit())
     with pytest.raises(NotImplementedError):
         sh.evolution_method.compile(sh.to_sim())
+
+
+def test_field_integrals_follow_the_reference_rules():
+    """pulses.py:59-98: rule='simps' (default) or 'trapz' for the integral and the vector potential,
+    cumtrapz with initial 0 for the cumulative forms, eps0 c int E^2 for the fluence."""
+    from scipy import integrate
+
+    p = potentials.SincPulse(pulse_width=80 * u.asec, fluence=2 * u.Jcm2, phase=0.3)
+    t = np.linspace(-300, 300, 601) * u.asec
+    e = p.get_electric_field_amplitude(t)
+    assert p.get_vector_potential_amplitude_numeric(t, rule="trapz") == pytest.approx(
+        -integrate.trapezoid(e, x=t), rel=1e-12)
+    assert p.get_vector_potential_amplitude_numeric(t) == pytest.approx(-integrate.simpson(e, x=t), rel=1e-12)
+    cum = p.get_vector_potential_amplitude_numeric_cumulative(t)
+    assert np.allclose(cum, -integrate.cumulative_trapezoid(e, x=t, initial=0), rtol=1e-12,
+                       atol=1e-12 * np.max(np.abs(cum)))
+    assert cum[0] == 0.0
+    assert p.get_fluence_numeric(t, rule="trapz") == pytest.approx(
+        u.epsilon_0 * u.c * integrate.trapezoid(e ** 2, x=t), rel=1e-12)
+    with pytest.raises(ValueError):
+        p.get_vector_potential_amplitude_numeric(t, rule="romb")
