@@ -1,0 +1,237 @@
+"""
+Every workload at its real size on the device, against the oracle wherever the oracle can follow:
+
+  line_cn    8192 points x 4000 steps: the whole run against the oracle's whole run
+             (tests/golden/config_cases.npz, written by make_config_golden.py).
+  sh_single  200 l x 1000 r x 8000 steps: the whole run against the oracle's whole run.
+  sh_scan    all 256 seeded draws (111 M mesh points) in one batch, every draw to its last step;
+             three draws (l_bound 283 and 143, which are odd, and 138; 2066-3874 radial points;
+             3.3-7.8 J/cm^2) checked against the oracle over the 100 steps around their pulse
+             peaks, starting from the device's own state there.
+  cyl_scan   all 32 Gaussian-pulse simulations of 2000 x 1000 points with their mask, every one to
+             its last step; the strongest-field one checked against the oracle over 16 steps around
+             its pulse peak.
+  served     the 70/20/10 request mix at real mesh sizes with short step counts: every result
+             served in the mix equals its result alone, bit for bit.
+
+Tolerances (the stated bar): final wavefunction relative L2 <= 1e-10; norm, inner products,
+bound-state overlap and ionization probability within 1e-9 absolute.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from ionization_amd import engine, serve, workloads
+from ionization_amd import units as u
+from oracle import mesh_cpu, physics
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config_cases.npz")
+
+
+@pytest.fixture(scope="module")
+def fx():
+    with np.load(FIXTURE) as f:
+        return {k: f[k] for k in f.files}
+
+
+def test_line_cn_8192x4000_whole_run_matches_oracle(fx):
+    spec = workloads.line_cn_spec("line", ground_state=fx["line_g0"], store_data_every=100)
+    sim = spec.to_sim()
+    assert sim.time_steps == 4001 and sim.mesh.g.shape == (8192,)
+    print("line_cn 8192 x 4000 on the device", flush=True)
+    sim.run()
+    err = rel_l2(sim.mesh.g, fx["line_g"])
+    dn = np.max(np.abs(sim.data.norm - fx["line_norms"]))
+    dev_ips = np.array([sim.data.inner_products[s] for s in spec.test_states]).T
+    dip = np.max(np.abs(dev_ips - fx["line_ips"]))
+    print(f"line_cn: rel L2 {err:.2e}, norm {dn:.2e}, inner product {dip:.2e}", flush=True)
+    assert err <= 1e-10 and dn <= 1e-9 and dip <= 1e-9
+    assert abs(fx["line_ips"][-1, 0]) ** 2 < 1 - 1e-3  # the field moved population out of the ground state
+
+
+def test_sh_single_200x1000x8000_whole_run_matches_oracle(fx):
+    spec = workloads.sh_single_spec("sh", store_data_every=100)
+    sim = spec.to_sim()
+    assert sim.time_steps == 8001 and sim.mesh.g.shape == (200, 1000)
+    assert len(spec.test_states) == fx["sh_ips"].shape[1]
+    # the same start vector as the oracle's (numeric eigenvector signs are arbitrary)
+    sim.mesh.g = np.array(fx["sh_g0"])
+    print("sh_single 200 x 1000 x 8000 on the device", flush=True)
+    sim.run()
+    err = rel_l2(sim.mesh.g, fx["sh_g"])
+    dn = np.max(np.abs(sim.data.norm - fx["sh_norms"]))
+    dev_ips = np.array([sim.data.inner_products[s] for s in spec.test_states]).T
+    # per-state sign of the numeric basis may differ between machines: compare magnitudes and overlaps
+    dip = np.max(np.abs(np.abs(dev_ips) - np.abs(fx["sh_ips"])))
+    bound = fx["sh_bound"]
+    ref_overlap = np.sum(np.abs(fx["sh_ips"][:, bound]) ** 2, axis=1)
+    dov = np.max(np.abs(sim.data.bound_state_overlap - ref_overlap))
+    dion = np.max(np.abs((sim.data.norm - sim.data.bound_state_overlap) - (fx["sh_norms"] - ref_overlap)))
+    dl = np.max(np.abs(sim.lnorm - fx["sh_lnorms"]))
+    print(f"sh_single: rel L2 {err:.2e}, norm {dn:.2e}, |inner product| {dip:.2e}, bound overlap {dov:.2e}, "
+          f"ionization {dion:.2e}, largest-l norm {dl:.2e}", flush=True)
+    assert err <= 1e-10 and dn <= 1e-9 and dip <= 1e-9 and dov <= 1e-9 and dion <= 1e-9 and dl <= 1e-9
+    assert ref_overlap[-1] < 1 - 1e-4  # the pulse ionized
+
+
+SCAN_CHECKED = (29, 239, 195)
+
+
+def _scan_oracle(p):
+    field = lambda t: physics.sinc_pulse_field(t, pulse_width=p["pulse_width"], fluence=p["fluence"],
+                                               phase=p["phase"])
+    rb = p["r_bound"]
+    return mesh_cpu.SHOracle(r_bound=rb, r_points=p["r_points"], l_bound=p["l_bound"], potential=physics.coulomb,
+                             field=field, mask=lambda r: physics.radial_cosine_mask(r, 0.9 * rb, rb))
+
+
+@pytest.mark.timeout(900)
+def test_sh_scan_all_256_draws_in_one_batch():
+    params = workloads.sh_scan_params(256, 0)
+    sims, comps = [], []
+    for p in params:
+        sim = workloads.sh_scan_spec(p).to_sim()
+        sims.append(sim)
+        comps.append(sim.spec.evolution_method.compile(sim))
+    points = sum(c.mesh_points for c in comps)
+    assert len(comps) == 256 and points > 1.1e8
+    chk = [params[k] for k in SCAN_CHECKED]
+    assert any(p["l_bound"] % 2 for p in chk) and max(p["r_points"] for p in chk) > 3800
+    pool = int(sum(c.footprint_bytes for c in comps) * 1.1) + (256 << 20)
+    print(f"sh_scan: 256 draws, {points / 1e6:.1f} M points, pool {pool / 2**30:.1f} GiB", flush=True)
+    with engine.DeviceContext(pool_bytes=pool) as ctx:
+        slots = [ctx.add_sh(c) for c in comps]
+        events = []
+        for k in SCAN_CHECKED:
+            s0 = comps[k].n_steps // 2 - 50
+            events += [(s0, k, "start"), (s0 + 100, k, "end")]
+        done, got = 0, {}
+        for rep, k, which in sorted(events):
+            ctx.run(rep - done)
+            done = rep
+            ctx.materialize(engine.KIND_SH, slots[k])
+            step = ctx.state(engine.KIND_SH, slots[k])[0]
+            got[(k, which)] = (step, ctx.download_sh(slots[k]).g)
+        total = max(c.n_steps for c in comps) + 1
+        while done < total:
+            n = min(4000, total - done)
+            ctx.run(n)
+            done += n
+            print(f"sh_scan: {done} / {total} replays", flush=True)
+        for i, (c, s) in enumerate(zip(comps, slots)):
+            assert ctx.finished(engine.KIND_SH, s), i
+            r = ctx.download_sh(s, with_g=False)
+            assert np.all(np.isfinite(r.norms)) and np.all(np.isfinite(r.ips)), i
+            assert np.all(np.diff(r.norms) <= 1e-12) and r.norms[0] == pytest.approx(1, abs=1e-12), i
+            overlap = np.sum(np.abs(r.ips) ** 2, axis=1)
+            assert np.all(overlap <= r.norms + 1e-12), i
+    for k in SCAN_CHECKED:
+        p, c, sim = params[k], comps[k], sims[k]
+        (s0, g_start), (s1, g_end) = got[(k, "start")], got[(k, "end")]
+        assert s1 - s0 == 100
+        print(f"sh_scan draw {k}: {p['l_bound']} x {p['r_points']}, oracle steps {s0}..{s1}", flush=True)
+        g_ref = _scan_oracle(p).run(g_start, sim.times[s0:s1 + 1], sim.spec.time_step, 100)[0]
+        basis = [np.real(sim.mesh.get_radial_g_for_state(s)) for s in sim.spec.test_states]
+        dr = c.delta_r
+        ov = lambda g: sum(abs(np.sum(b * g[st.l]) * dr) ** 2 for b, st in zip(basis, sim.spec.test_states))
+        nrm = lambda g: float(np.sum(np.abs(g) ** 2) * dr)
+        err = rel_l2(g_end, g_ref)
+        print(f"  rel L2 {err:.2e}, norm {abs(nrm(g_end) - nrm(g_ref)):.2e}, "
+              f"bound overlap {abs(ov(g_end) - ov(g_ref)):.2e}, the window moved g by {rel_l2(g_end, g_start):.2e}",
+              flush=True)
+        assert err <= 1e-10
+        assert abs(nrm(g_end) - nrm(g_ref)) <= 1e-9 and abs(ov(g_end) - ov(g_ref)) <= 1e-9
+        assert rel_l2(g_end, g_start) > 1e-3  # the window does something
+
+
+CYL_STEPS = 16
+
+
+@pytest.mark.timeout(900)
+def test_cyl_scan_all_32_gaussian_pulse_simulations_with_mask():
+    params = workloads.cyl_scan_params(32, 0)
+    comps = workloads.cyl_scan_compiled(params)
+    assert len(comps) == 32 and all(c.z_points == 2000 and c.rho_points == 1000 for c in comps)
+    k = int(np.argmax([p["amplitude"] for p in params]))
+    p = params[k]
+    spec = workloads.cyl_scan_spec(p)
+    sim = spec.to_sim()
+    pool = int(sum(c.footprint_bytes for c in comps) * 1.1) + (256 << 20)
+    print(f"cyl_scan: 32 x 2000 x 1000, pool {pool / 2**30:.1f} GiB; simulation {k}: "
+          f"E0 {p['amplitude'] / u.atomic_electric_field:.3f} au, sigma {p['pulse_width'] / u.asec:.0f} as",
+          flush=True)
+    with engine.DeviceContext(pool_bytes=pool) as ctx:
+        slots = [ctx.add_cyl(c) for c in comps]
+        ctx.set_hold(engine.KIND_CYL, slots[k], True)
+        s0 = comps[k].n_steps // 2 - CYL_STEPS // 2
+        ctx.run(s0)
+        g_start = ctx.download_cyl(slots[k])[0]
+        assert ctx.state(engine.KIND_CYL, slots[k])[0] == s0
+        ctx.run(CYL_STEPS)
+        g_end = ctx.download_cyl(slots[k])[0]
+        done = s0 + CYL_STEPS
+        total = max(c.n_steps for c in comps)
+        while done < total:
+            n = min(2000, total - done)
+            ctx.run(n)
+            done += n
+            print(f"cyl_scan: {done} / {total} steps", flush=True)
+        for i, (c, s) in enumerate(zip(comps, slots)):
+            assert ctx.finished(engine.KIND_CYL, s), i
+            _, norms, ips = ctx.download_cyl(s, with_g=False)
+            assert np.all(np.isfinite(norms)) and np.all(np.isfinite(ips)), i
+            # the ADI step is S^-1 (C_z C_rho) S with S = 1 + i tau H_rho(t): with a time-dependent field it
+            # is not unitary, and the norm moves by up to ~1e-5 under the strongest pulses of this scan
+            assert norms[0] == pytest.approx(1, abs=1e-9) and np.all(norms <= 1 + 1e-4), i
+    orc = mesh_cpu.CylOracle(
+        z_bound=spec.z_bound, rho_bound=spec.rho_bound, z_points=spec.z_points, rho_points=spec.rho_points,
+        potential=physics.coulomb,
+        field=lambda t: physics.gaussian_pulse_field(t, pulse_width=p["pulse_width"], amplitude=p["amplitude"]),
+        mask=lambda r: physics.radial_cosine_mask(r, 45 * u.bohr_radius, 50 * u.bohr_radius))
+    print(f"cyl_scan simulation {k}: oracle steps {s0}..{s0 + CYL_STEPS}", flush=True)
+    g_ref = orc.run(g_start, sim.times[s0:s0 + CYL_STEPS + 1], CYL_STEPS)[0]
+    err = rel_l2(g_end, g_ref)
+    w = orc.delta_z * orc.delta_rho
+    dn = abs(np.sum(np.abs(g_end) ** 2) - np.sum(np.abs(g_ref) ** 2)) * w
+    print(f"  rel L2 {err:.2e}, norm {dn:.2e}, the window moved g by {rel_l2(g_end, g_start):.2e}", flush=True)
+    assert err <= 1e-10 and dn <= 1e-9
+    assert rel_l2(g_end, g_start) > 1e-4
+
+
+def _alone(c):
+    with engine.DeviceContext(pool_bytes=int(c.footprint_bytes * 1.5) + (64 << 20)) as ctx:
+        if isinstance(c, engine.CompiledSH):
+            s = ctx.add_sh(c)
+            ctx.run(c.n_steps + 1)
+            r = ctx.download_sh(s)
+            return r.g, r.norms, r.ips
+        s = ctx.add_line(c)
+        ctx.run(c.n_steps)
+        return ctx.download_line(s)
+
+
+@pytest.mark.timeout(900)
+def test_served_mix_at_real_mesh_sizes_equals_alone():
+    reqs = serve.served_mix(24, rate_per_s=200.0, seed=3, single_steps=300, scan_steps=300, line_steps=300)
+    kinds = {r.kind for r in reqs}
+    assert kinds == {"sh_single", "sh_scan", "line_cn"}, kinds
+    comps = {r.rid: r.make() for r in reqs}
+    for r in reqs:
+        r.make = (lambda c=comps[r.rid]: c)
+    assert all(c.n_steps <= 301 for c in comps.values())
+    pool = max(serve.Server._reserve(c) for c in comps.values()) * 3
+    print(f"served: {len(reqs)} requests ({sorted(kinds)}), pool {pool / 2**30:.2f} GiB", flush=True)
+    with serve.Server(pool_bytes=pool, chunk_steps=25) as srv:
+        rep = srv.run(reqs, max_seconds=600)
+    assert len(rep.requests) == len(reqs)
+    print(f"served: p50 {rep.p50:.2f} s, p99 {rep.p99:.2f} s, {rep.rebuilds} graph rebuilds", flush=True)
+    for r in rep.requests:
+        g, norms, ips = _alone(comps[r.rid])
+        assert np.array_equal(r.g, g), (r.rid, r.kind)
+        assert np.array_equal(r.norms, norms), (r.rid, r.kind)
+        assert np.array_equal(r.ips, ips), (r.rid, r.kind)
