@@ -10,7 +10,14 @@ graph), complex128, on one MI355X.
     python bench.py --summarize-profiles DIR  (CPU: rocprofv3 databases -> profiles/)
 
 Prints one JSON line. ``value`` is the batched throughput with every
-wavefunction resident in HBM; ``roofline`` is for the kernel with the largest
+wavefunction resident in HBM, timed over a window in which every simulation
+is at its pulse peak and takes one data step: each simulation starts (from its
+initial state, uploaded at that step of its own time grid) so that the timed
+steps are centred on its data time nearest its pulse peak. Beside it: the same
+at store_data_every = 1 (the reference default) centred on the peak, the first
+steps of every run (weak fields, no data step), and the whole scan run to its
+end. ``plain_value`` times the 200 x 1000 simulation alone over the same
+window relative to its peak, at the same cadence. ``roofline`` is for the kernel with the largest
 share of device time, from HIP events recorded on the library's stream
 around every launch of a profiled pass over K more steps; ``cpu_baseline``
 is the CPU oracle (a restatement of the reference's evolve step) timed on
@@ -53,13 +60,33 @@ def build_scan(n_sims, seed, store_every):
     return params, compiled
 
 
-def time_batched(compiled, steps, warmup):
-    from ionization_amd.engine import DeviceContext
+def peak_starts(compiled, steps, warmup, store_every):
+    """
+    First step of each simulation for a timed window centred on its pulse peak (t = 0, the middle
+    of its symmetric time grid): on the data time nearest the peak when data are stored every
+    `store_every` > 1 steps (so the window holds exactly one data step, at most 50 steps from the
+    peak for a cadence of 100), on the peak itself at a cadence of 1. The warm-up steps come first.
+    """
+    out = []
+    for c in compiled:
+        peak = c.n_steps // 2
+        centre = int(round(peak / store_every)) * store_every if store_every > 1 else peak
+        out.append(max(0, centre - steps // 2 - warmup))
+    return out
+
+
+def time_batched(compiled, steps, warmup, starts=None):
+    """Graph-replayed steps of every simulation at once; `starts` puts each simulation's initial
+    state at that step of its time grid first (see peak_starts)."""
+    from ionization_amd.engine import DeviceContext, KIND_SH, from_positions
 
     pool = int(sum(c.footprint_bytes for c in compiled) * 1.1) + (256 << 20)
     ctx = DeviceContext(pool_bytes=pool)
-    for c in compiled:
-        ctx.add_sh(c)
+    slots = [ctx.add_sh(c) for c in compiled]
+    if starts is not None:
+        for c, s, st in zip(compiled, slots, starts):
+            if st > 0:
+                ctx.upload_state(KIND_SH, s, from_positions(c.g0, c.r_points, c.P, c.m), st)
     points = sum(c.mesh_points for c in compiled)
     ctx.run(warmup)
     ctx.sync()
@@ -97,7 +124,7 @@ def time_velocity(params, store_every, steps, warmup):
                                      operators=mesh.SphericalHarmonicVelocityGaugeOperators()).to_sim()
         compiled.append(sim.spec.evolution_method.compile(sim))
     build_s = time.perf_counter() - t
-    b = time_batched(compiled, steps, warmup)
+    b = time_batched(compiled, steps, warmup, peak_starts(compiled, steps, warmup, store_every))
     rate = b["points"] * steps / b["wall_s"]
     prof = b["prof"]
     # h2 sweeps of k_sh_vphase read and write the rows inside l-pairs, before and after the solve:
@@ -109,7 +136,8 @@ def time_velocity(params, store_every, steps, warmup):
     bytes_step = 64 * pts + h2_bytes  # L-phase and R-phase sweeps plus the separate h2 sweeps
     h2_ms = prof["vphase_ms"] / steps
     out = {
-        "workload": f"sh_scan draws ({len(compiled)} simulations), velocity gauge, one hipGraph replay per step",
+        "workload": f"sh_scan draws ({len(compiled)} simulations), velocity gauge, one hipGraph replay per step, "
+                    "the window on each simulation's data time nearest its pulse peak",
         "value": rate, "unit": "mesh-point-steps/s", "ms_per_step": b["wall_s"] * 1e3 / steps,
         "bytes_per_point_step": bytes_step / pts, "step_hbm_gbs": rate * (bytes_step / pts) / 1e9, "build_s": build_s,
         "kernels_ms_per_step": {"k_sh_lphase": prof["lphase_ms"] / steps, "k_sh_rphase": prof["rphase_ms"] / steps,
@@ -137,10 +165,11 @@ def time_sh_adi(params, store_every, steps, warmup):
                                      evolution_method=mesh.AlternatingDirectionImplicit()).to_sim()
         compiled.append(sim.spec.evolution_method.compile(sim))
     build_s = time.perf_counter() - t
-    b = time_batched(compiled, steps, warmup)
+    b = time_batched(compiled, steps, warmup, peak_starts(compiled, steps, warmup, store_every))
     rate = b["points"] * steps / b["wall_s"]
     out = {"workload": f"sh_scan draws ({len(compiled)} simulations), AlternatingDirectionImplicit, one hipGraph "
-                       "replay per step", "value": rate, "unit": "mesh-point-steps/s",
+                       "replay per step, the window on each simulation's data time nearest its pulse peak",
+           "value": rate, "unit": "mesh-point-steps/s",
            "ms_per_step": b["wall_s"] * 1e3 / steps, "build_s": build_s}
     log(f"SH ADI: {out['ms_per_step']:.3f} ms/step, {rate / 1e9:.2f} G mesh-point-steps/s")
     return out
@@ -216,22 +245,68 @@ def time_reference_specs(cpu_seconds=0.0):
     return out
 
 
-def time_plain(steps, warmup):
-    from ionization_amd import workloads
+def time_full_run(compiled, build_cpu_s):
+    """
+    The whole scan: every simulation from its first to its last step in one batch (the batch thins
+    out as the shorter runs finish). Also counted as the reference benchmark counts
+    (benchmarks/benchmark_mesh_evolution.py:14-23, 36): len(times) x mesh points of every
+    simulation over the process CPU time of building and running them.
+    """
     from ionization_amd.engine import DeviceContext
 
-    sim = workloads.sh_single_spec(store_data_every=100).to_sim()
-    comp = sim.spec.evolution_method.compile(sim)
-    ctx = DeviceContext(pool_bytes=comp.footprint_bytes * 2 + (64 << 20))
-    ctx.add_sh(comp)
-    ctx.run(warmup, use_graph=False)
+    pool = int(sum(c.footprint_bytes for c in compiled) * 1.1) + (256 << 20)
+    c0 = time.process_time()
+    ctx = DeviceContext(pool_bytes=pool)
+    for c in compiled:
+        ctx.add_sh(c)
+    total = max(c.n_steps for c in compiled) + 1
+    work = sum(c.mesh_points * c.n_steps for c in compiled)
     ctx.sync()
     t0 = time.perf_counter()
-    ctx.run(steps, use_graph=False)
+    done = 0
+    while done < total:
+        k = min(4000, total - done)
+        ctx.run(k)
+        done += k
     ctx.sync()
     wall = time.perf_counter() - t0
+    cpu = time.process_time() - c0
     ctx.close()
-    return dict(points=comp.mesh_points, wall_s=wall)
+    spacetime = sum(c.mesh_points * (c.n_steps + 1) for c in compiled)
+    out = {"workload": f"the {len(compiled)} scan simulations, each from its first to its last step in one batch",
+           "value": work / wall, "unit": "mesh-point-steps/s", "wall_s": wall, "replays": total,
+           "reference_convention": {"spacetime_points": spacetime, "cpu_s": build_cpu_s + cpu,
+                                    "pts_per_cpu_s": spacetime / (build_cpu_s + cpu)}}
+    log(f"full run: {total} replays in {wall:.1f} s, {work / wall / 1e9:.2f} G mesh-point-steps/s; "
+        f"{spacetime / (build_cpu_s + cpu) / 1e9:.2f} G spacetime points per CPU second with setup")
+    return out
+
+
+def time_plain(steps, warmup, store_every, repeats=5):
+    """sh_single (200 l x 1000 r) alone with direct launches, over the window peak_starts gives it at
+    the same cadence as the batch; the median of `repeats` passes over that window."""
+    import dataclasses
+
+    from ionization_amd import workloads
+    from ionization_amd.engine import DeviceContext, KIND_SH, from_positions
+
+    sim = workloads.sh_single_spec(store_data_every=store_every).to_sim()
+    comp = dataclasses.replace(sim.spec.evolution_method.compile(sim), store_every=store_every)
+    start = peak_starts([comp], steps, warmup, store_every)[0]
+    g0 = from_positions(comp.g0, comp.r_points, comp.P, comp.m)
+    ctx = DeviceContext(pool_bytes=comp.footprint_bytes * 2 + (64 << 20))
+    slot = ctx.add_sh(comp)
+    walls = []
+    for _ in range(repeats):
+        ctx.upload_state(KIND_SH, slot, g0, start)
+        ctx.run(warmup, use_graph=False)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.run(steps, use_graph=False)
+        ctx.sync()
+        walls.append(time.perf_counter() - t0)
+    ctx.close()
+    return dict(points=comp.mesh_points, wall_s=float(np.median(walls)), start=start)
 
 
 def time_cyl(n_sims, seed, steps, warmup):
@@ -335,17 +410,25 @@ def time_line(n_batch, cpu_seconds):
     return out
 
 
-def time_serve(n, rate, seed, pool_gb):
+def time_serve(n, rate, seed, pool_gb, fallback_pool_gb=96.0):
     """
     The served stream: Poisson arrivals (seeded) of 70 % sh_single-like, 20 %
     sh_scan draws and 10 % line_cn requests; compile on arrival, admission by
-    device footprint, retirement as soon as the last step has run.
+    device footprint, retirement as soon as the last step has run. The admission
+    pool is `pool_gb` GiB of the GPU's HBM when it can be reserved, else the fallback.
     """
     from ionization_amd import serve
+    from ionization_amd.exceptions import DeviceError
 
     gs = serve._line_ground_state()
     reqs = serve.served_mix(n, rate, seed, line_ground_state=gs)
-    with serve.Server(pool_bytes=int(pool_gb * (1 << 30)), chunk_steps=25, compile_workers=8) as srv:
+    try:
+        srv = serve.Server(pool_bytes=int(pool_gb * (1 << 30)), chunk_steps=25, compile_workers=8)
+    except DeviceError as e:
+        log(f"serve: a {pool_gb} GiB pool could not be reserved ({e}); using {fallback_pool_gb} GiB")
+        pool_gb = fallback_pool_gb
+        srv = serve.Server(pool_bytes=int(pool_gb * (1 << 30)), chunk_steps=25, compile_workers=8)
+    with srv:
         rep = srv.run(reqs)
     kinds = {}
     for r in rep.requests:
@@ -354,11 +437,31 @@ def time_serve(n, rate, seed, pool_gb):
                        "(200 x 1000, 8000 steps), 20 % sh_scan draws, 10 % line_cn (8192 x 4000); "
                        f"{pool_gb} GiB admission pool",
            "value": rep.throughput, "unit": "mesh-point-steps/s", "p50_s": rep.p50, "p99_s": rep.p99,
-           "makespan_s": rep.makespan_s, "replays": rep.replays,
+           "makespan_s": rep.makespan_s, "replays": rep.replays, "graph_rebuilds": rep.rebuilds,
+           "pool_gib": pool_gb, "errors": sum(r.error is not None for r in rep.requests),
            "by_kind": {k: {"n": len(v), "p50_s": float(np.percentile(v, 50)), "max_s": float(np.max(v))}
                        for k, v in kinds.items()}}
-    log(f"serve: {n} requests, p50 {rep.p50:.2f} s, p99 {rep.p99:.2f} s, {rep.throughput / 1e9:.2f} G/s")
+    log(f"serve: {n} requests, p50 {rep.p50:.2f} s, p99 {rep.p99:.2f} s, {rep.throughput / 1e9:.2f} G/s, "
+        f"{rep.rebuilds} graph rebuilds over {rep.replays} replays, {pool_gb} GiB pool")
     return out
+
+
+def host_info():
+    """The host the CPU baselines ran on: its logical CPUs, the ones this process may use, the model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "cpu_model": model}
 
 
 def cpu_baseline(params, seconds):
@@ -642,7 +745,8 @@ def main():
     ap.add_argument("--sims", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--store-every", type=int, default=100)
-    ap.add_argument("--plain-steps", type=int, default=400)
+    ap.add_argument("--plain-steps", type=int, default=1,
+                    help="0 skips the plain path (one simulation alone, timed over the same window)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-procs", type=int, default=16,
                     help="processes for the parallel CPU baseline (the host share of one GPU)")
@@ -659,7 +763,9 @@ def main():
     ap.add_argument("--line-batch", type=int, default=256, help="0 skips the LineMesh workload")
     ap.add_argument("--serve-requests", type=int, default=96, help="0 skips the served stream")
     ap.add_argument("--serve-rate", type=float, default=20.0)
-    ap.add_argument("--serve-pool-gb", type=float, default=96.0)
+    ap.add_argument("--serve-pool-gb", type=float, default=256.0,
+                    help="admission pool (GiB of HBM); 96 GiB if that much cannot be reserved")
+    ap.add_argument("--full-run", type=int, default=1, help="0 skips running the whole scan to its end")
     ap.add_argument("--summarize-profiles", metavar="DIR")
     ap.add_argument("--summarize-velocity-profiles", metavar="DIR")
     ap.add_argument("--trace-command", default="")
@@ -680,8 +786,9 @@ def main():
               "simulations with no exchange, and the whole-job rate is the sum", file=sys.stderr)
         sys.exit(2)
 
-    t = time.perf_counter()
+    t, c0 = time.perf_counter(), time.process_time()
     params, compiled = build_scan(args.sims, args.seed, args.store_every)
+    build_cpu_s = time.process_time() - c0
     log(f"built {len(compiled)} simulations in {time.perf_counter() - t:.1f} s; "
         f"{sum(c.mesh_points for c in compiled) / 1e6:.1f} M mesh points")
     cpu_par = None
@@ -690,14 +797,32 @@ def main():
         cpu_par = {"value": cp["value"], "unit": "mesh-point-steps/s", "cores": cp["procs"], "kind": "port",
                    "sample": cp["sample"]}
         log(f"cpu oracle, {cp['procs']} processes: {cp['value'] / 1e6:.2f} M mesh-point-steps/s")
-    b = time_batched(compiled, args.steps, args.warmup)
+    import dataclasses
+
+    starts = peak_starts(compiled, args.steps, args.warmup, args.store_every)
+    b = time_batched(compiled, args.steps, args.warmup, starts)
     rate = b["points"] * args.steps / b["wall_s"]
-    log(f"batched: {b['wall_s'] * 1e3 / args.steps:.3f} ms/step, {rate / 1e9:.2f} G mesh-point-steps/s")
-    plain_rate = None
+    log(f"batched, window on each data time nearest the pulse peak: {b['wall_s'] * 1e3 / args.steps:.3f} ms/step, "
+        f"{rate / 1e9:.2f} G mesh-point-steps/s")
+    every1 = [dataclasses.replace(c, store_every=1) for c in compiled]
+    b1 = time_batched(every1, args.steps, args.warmup, peak_starts(every1, args.steps, args.warmup, 1))
+    rate1 = b1["points"] * args.steps / b1["wall_s"]
+    log(f"batched, store_data_every = 1, window on the pulse peak: {b1['wall_s'] * 1e3 / args.steps:.3f} ms/step, "
+        f"{rate1 / 1e9:.2f} G mesh-point-steps/s")
+    del every1
+    b0 = time_batched(compiled, args.steps, args.warmup)
+    rate0 = b0["points"] * args.steps / b0["wall_s"]
+    log(f"batched, first steps of every run: {b0['wall_s'] * 1e3 / args.steps:.3f} ms/step, "
+        f"{rate0 / 1e9:.2f} G mesh-point-steps/s")
+    plain_rate = plain_rate1 = None
     if args.plain_steps > 0:  # 0 skips the plain pass (profiling runs)
-        pl = time_plain(args.plain_steps, 10)
-        plain_rate = pl["points"] * args.plain_steps / pl["wall_s"]
-        log(f"plain: {pl['wall_s'] * 1e3 / args.plain_steps:.3f} ms/step, {plain_rate / 1e9:.3f} G mesh-point-steps/s")
+        pl = time_plain(args.steps, args.warmup, args.store_every)
+        plain_rate = pl["points"] * args.steps / pl["wall_s"]
+        pl1 = time_plain(args.steps, args.warmup, 1)
+        plain_rate1 = pl1["points"] * args.steps / pl1["wall_s"]
+        log(f"plain, same window: {pl['wall_s'] * 1e3 / args.steps:.3f} ms/step, {plain_rate / 1e9:.3f} G "
+            f"mesh-point-steps/s; at store_data_every = 1 {plain_rate1 / 1e9:.3f} G")
+    full = time_full_run(compiled, build_cpu_s) if args.full_run else None
 
     tuned = None
     if args.autotune:
@@ -746,9 +871,13 @@ def main():
     copy_gbs = calibrate_copy()  # attainable streaming rate, beside the published peak
 
     cpu = None
+    host = host_info()
+    if cpu_par is not None:
+        cpu_par["host"] = host
     if args.cpu_seconds > 0:
         c = cpu_baseline(params, args.cpu_seconds)
-        cpu = {"value": c["value"], "unit": "mesh-point-steps/s", "cores": 1, "kind": "port", "sample": c["sample"]}
+        cpu = {"value": c["value"], "unit": "mesh-point-steps/s", "cores": 1, "kind": "port", "sample": c["sample"],
+               "host": host}
         log(f"cpu oracle: {c['value'] / 1e6:.2f} M mesh-point-steps/s ({c['sample']})")
 
     out = {
@@ -766,14 +895,26 @@ def main():
         "data": "synthetic (seeded parameter draws; numeric 1s initial states)",
         "config": {
             "workload": f"sh_scan: {args.sims} SphericalHarmonicMesh simulations, seed {args.seed}, "
-                        "split-operator length gauge, dt = 1 as, one hipGraph replay per step",
+                        "split-operator length gauge, dt = 1 as, one hipGraph replay per step; each simulation "
+                        "timed over the steps around its data time nearest its pulse peak",
             "simulations": args.sims,
             "mesh_points": b["points"],
             "store_data_every": args.store_every,
-            "plain_workload": "sh_single: 200 l x 1000 r, one simulation, direct launches, no graph",
+            "window": "each simulation starts from its initial state at step (d - steps // 2 - warmup) of its own "
+                      "time grid, d = its data time nearest the pulse peak",
+            "plain_workload": "sh_single: 200 l x 1000 r, one simulation, direct launches, no graph, the same "
+                              "window relative to its peak and the same cadence",
         },
         "plain_value": plain_rate,
         "batched_over_plain": None if plain_rate is None else rate / plain_rate,
+        "store_data_every_1": {"value": rate1, "ms_per_step": b1["wall_s"] * 1e3 / args.steps,
+                               "plain_value": plain_rate1,
+                               "batched_over_plain": None if plain_rate1 is None else rate1 / plain_rate1,
+                               "window": "centred on each simulation's pulse peak; every step is a data step"},
+        "first_steps_of_each_run": {"value": rate0, "ms_per_step": b0["wall_s"] * 1e3 / args.steps,
+                                    "window": f"steps {args.warmup}..{args.warmup + args.steps - 1} of every run "
+                                              "(weak fields, no data step)"},
+        "full_run": full,
         "device_ms_per_step": b["dev_ms"] / args.steps,
         "step_hbm_gbs": step_gbs,
         "step_hbm_frac": step_gbs / HBM_PEAK_GBS,
