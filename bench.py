@@ -568,7 +568,8 @@ def summarize_profiles(d, command_trace, command_fetch, command_write, trace_ste
         f = _kernel_family(name)
         c, t = fam.get(f, (0, 0.0))
         fam[f] = (c + cnt, t + tot)
-    lines = ["# rocprofv3 kernel trace of bench.py: the 256-simulation SphericalHarmonicMesh scan and the",
+    lines = ["# rocprofv3 kernel trace of bench.py: the 256-simulation SphericalHarmonicMesh scan over its",
+             "# headline window (each simulation's steps around its data time nearest its pulse peak) and the",
              "# 32-simulation CylindricalSliceMesh workload", "",
              "command:", "    " + command_trace, "",
              f"{'kernel':<70} {'calls':>7} {'total ms':>10} {'avg us':>9} {'min us':>9} {'max us':>9}"]
@@ -766,6 +767,8 @@ def main():
     ap.add_argument("--serve-pool-gb", type=float, default=256.0,
                     help="admission pool (GiB of HBM); 96 GiB if that much cannot be reserved")
     ap.add_argument("--full-run", type=int, default=1, help="0 skips running the whole scan to its end")
+    ap.add_argument("--windows", choices=("all", "peak"), default="all",
+                    help="peak: only the headline window (profiling runs)")
     ap.add_argument("--summarize-profiles", metavar="DIR")
     ap.add_argument("--summarize-velocity-profiles", metavar="DIR")
     ap.add_argument("--trace-command", default="")
@@ -804,16 +807,23 @@ def main():
     rate = b["points"] * args.steps / b["wall_s"]
     log(f"batched, window on each data time nearest the pulse peak: {b['wall_s'] * 1e3 / args.steps:.3f} ms/step, "
         f"{rate / 1e9:.2f} G mesh-point-steps/s")
-    every1 = [dataclasses.replace(c, store_every=1) for c in compiled]
-    b1 = time_batched(every1, args.steps, args.warmup, peak_starts(every1, args.steps, args.warmup, 1))
-    rate1 = b1["points"] * args.steps / b1["wall_s"]
-    log(f"batched, store_data_every = 1, window on the pulse peak: {b1['wall_s'] * 1e3 / args.steps:.3f} ms/step, "
-        f"{rate1 / 1e9:.2f} G mesh-point-steps/s")
-    del every1
-    b0 = time_batched(compiled, args.steps, args.warmup)
-    rate0 = b0["points"] * args.steps / b0["wall_s"]
-    log(f"batched, first steps of every run: {b0['wall_s'] * 1e3 / args.steps:.3f} ms/step, "
-        f"{rate0 / 1e9:.2f} G mesh-point-steps/s")
+    every1_out = first_out = None
+    if args.windows == "all":  # "peak" (profiling runs) launches the headline window's kernels only
+        every1 = [dataclasses.replace(c, store_every=1) for c in compiled]
+        b1 = time_batched(every1, args.steps, args.warmup, peak_starts(every1, args.steps, args.warmup, 1))
+        rate1 = b1["points"] * args.steps / b1["wall_s"]
+        log(f"batched, store_data_every = 1, window on the pulse peak: {b1['wall_s'] * 1e3 / args.steps:.3f} "
+            f"ms/step, {rate1 / 1e9:.2f} G mesh-point-steps/s")
+        del every1
+        every1_out = {"value": rate1, "ms_per_step": b1["wall_s"] * 1e3 / args.steps,
+                      "window": "centred on each simulation's pulse peak; every step is a data step"}
+        b0 = time_batched(compiled, args.steps, args.warmup)
+        rate0 = b0["points"] * args.steps / b0["wall_s"]
+        log(f"batched, first steps of every run: {b0['wall_s'] * 1e3 / args.steps:.3f} ms/step, "
+            f"{rate0 / 1e9:.2f} G mesh-point-steps/s")
+        first_out = {"value": rate0, "ms_per_step": b0["wall_s"] * 1e3 / args.steps,
+                     "window": f"steps {args.warmup}..{args.warmup + args.steps - 1} of every run "
+                               "(weak fields, no data step)"}
     plain_rate = plain_rate1 = None
     if args.plain_steps > 0:  # 0 skips the plain pass (profiling runs)
         pl = time_plain(args.steps, args.warmup, args.store_every)
@@ -822,7 +832,10 @@ def main():
         plain_rate1 = pl1["points"] * args.steps / pl1["wall_s"]
         log(f"plain, same window: {pl['wall_s'] * 1e3 / args.steps:.3f} ms/step, {plain_rate / 1e9:.3f} G "
             f"mesh-point-steps/s; at store_data_every = 1 {plain_rate1 / 1e9:.3f} G")
-    full = time_full_run(compiled, build_cpu_s) if args.full_run else None
+    if every1_out is not None:
+        every1_out["plain_value"] = plain_rate1
+        every1_out["batched_over_plain"] = None if plain_rate1 is None else every1_out["value"] / plain_rate1
+    full =time_full_run(compiled, build_cpu_s) if args.full_run else None
 
     tuned = None
     if args.autotune:
@@ -907,13 +920,8 @@ def main():
         },
         "plain_value": plain_rate,
         "batched_over_plain": None if plain_rate is None else rate / plain_rate,
-        "store_data_every_1": {"value": rate1, "ms_per_step": b1["wall_s"] * 1e3 / args.steps,
-                               "plain_value": plain_rate1,
-                               "batched_over_plain": None if plain_rate1 is None else rate1 / plain_rate1,
-                               "window": "centred on each simulation's pulse peak; every step is a data step"},
-        "first_steps_of_each_run": {"value": rate0, "ms_per_step": b0["wall_s"] * 1e3 / args.steps,
-                                    "window": f"steps {args.warmup}..{args.warmup + args.steps - 1} of every run "
-                                              "(weak fields, no data step)"},
+        "store_data_every_1": every1_out,
+        "first_steps_of_each_run": first_out,
         "full_run": full,
         "device_ms_per_step": b["dev_ms"] / args.steps,
         "step_hbm_gbs": step_gbs,
