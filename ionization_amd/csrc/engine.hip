@@ -498,7 +498,8 @@ void dispatch_adi_lp(tdse_ctx* ctx, const Bucket& b) {
 
 template <int M, int WPE>
 void launch_rphase_vpair(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
-    const size_t lds = 3 * (size_t)b.P * 2 * sizeof(double2);
+    // the reduced system (3 P) and alpha_i of rows 0 .. M - 3 (M - 2 P) per line, two lines
+    const size_t lds = (size_t)(3 + M - 2) * b.P * 2 * sizeof(double2);
     hipLaunchKernelGGL((k_sh_rphase_vpair<M, kMaxPairThreads, WPE>), dim3((unsigned)b.fpairs.count), dim3(2 * b.P),
                        lds, st, ctx->d_sh, b.fpairs.ptr, (int)b.fpairs.count, ctx->d_cl);
 }

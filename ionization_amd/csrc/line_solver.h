@@ -294,19 +294,35 @@ __device__ __forceinline__ void cn_cayley_solve(double2 (&x)[M], const int k, co
 // as rsub_i = g_i (hm_i / hp_i), with ratio(i) = hm_i / hp_i (0 where hm_i = 0):
 // a product instead of rebuilding den_i and its reciprocal. Row ib of the thread
 // (the one row of the line with hp = 0 and hm != 0, or -1) is rebuilt as before.
+//
+// ALPHA_LDS (fewer instructions and registers; used where the two-line kernel
+// would otherwise spill):
+//  * pass 1 forms the pivots from the chunk's leading principal minors (no
+//    division in the recurrence) and stores alpha_i, i < m - 2, in LDS after
+//    the reduced system's 3 P LANES entries, at (3 P + i P + k) LANES + ln
+//    (M - 2 more P LANES double2; alpha_{m-2} stays in its register);
+//  * the row-0 coefficients through (L_k, Y_k) come from a backward sweep over
+//    the stored alpha_i instead of the running forward sums;
+//  * pass 2 disappears: pass 3 reads alpha_i back and forms
+//    z_i = delta_i - alpha_i L_k - gamma_i z_{i+1} (coef2 is not called).
+// Every thread reads back only what it wrote, so no barrier is needed.
 struct NoRatio {};
 
-template <int M, class Coef, class Coef2, bool EXPLICIT = false, int LANES = 1, class Ratio = NoRatio>
+template <int M, class Coef, class Coef2, bool EXPLICIT = false, int LANES = 1, class Ratio = NoRatio,
+          bool ALPHA_LDS = false>
 __device__ __forceinline__ void cn_implicit_lowreg(double2 (&x)[M], const int k, const int P, Coef coef,
                                                    Coef2 coef2, double2* __restrict__ lds,
                                                    const double2 xl = make_double2(0.0, 0.0),
                                                    const double2 xr = make_double2(0.0, 0.0), const int ln = 0,
                                                    Ratio ratio = Ratio{}, const int ib = -1) {
+    double2* __restrict__ alds = lds + (3 * P + k) * LANES + ln;  // alpha_i at alds[i P LANES]
     double2 ga[M - 1], de[M - 1];
     double2 al = cmk(0.0, 0.0), pi = cmk(1.0, 0.0), sa = cmk(0.0, 0.0), sd = cmk(0.0, 0.0);
     double2 gprev = cmk(0.0, 0.0), dprev = cmk(0.0, 0.0);
     double2 dlast = cmk(0.0, 0.0), diaglast = cmk(1.0, 0.0);
     double sublast = 0.0, suplast = 0.0;
+    double2 Dm1 = cmk(1.0, 0.0), Dm2 = cmk(0.0, 0.0);  // ALPHA_LDS: D_{i-1}, D_{i-2}
+    double hpp = 0.0;                                  // ALPHA_LDS: hp_{i-1}
 #pragma unroll
     for (int i = 0; i < M; ++i) {
         double hm, h, hp;
@@ -321,16 +337,38 @@ __device__ __forceinline__ void cn_implicit_lowreg(double2 (&x)[M], const int k,
             rhs = cmk(x[i].x + hx.y, x[i].y - hx.x);
         }
         if (i < M - 1) {
-            // den = (1 + i h) - (i hm) gamma_{i-1}
-            const double2 den = (i == 0) ? cmk(1.0, h) : cmk(fma(hm, gprev.y, 1.0), fma(-hm, gprev.x, h));
-            const double2 r = crecip(den);
+            double2 r;
+            if constexpr (ALPHA_LDS) {
+                // The pivots den_i = (1 + i h) - (i hm) gamma_{i-1} are ratios D_i / D_{i-1} of the
+                // chunk's leading principal minors, D_i = (1 + i h_i) D_{i-1} + hm_i hp_{i-1} D_{i-2}
+                // (D_{-1} = 1, D_{-2} = 0): the recurrence carries no division, and
+                // r_i = D_{i-1} / D_i is formed off it. |D_i| >= |D_{i-1}|: every pivot has
+                // real part >= 1 (the Hermitian part of 1 + i H, H real symmetric, is the
+                // identity, and Schur complements keep that bound), so nothing underflows.
+                const double c = hm * hpp;
+                const double2 Di = cmk(fma(c, Dm2.x, fma(-h, Dm1.y, Dm1.x)), fma(c, Dm2.y, fma(h, Dm1.x, Dm1.y)));
+                const double inv = tdse_rcp(fma(Di.x, Di.x, Di.y * Di.y));
+                r = cmk(fma(Dm1.x, Di.x, Dm1.y * Di.y) * inv, fma(Dm1.y, Di.x, -Dm1.x * Di.y) * inv);
+                Dm2 = Dm1;
+                Dm1 = Di;
+                hpp = hp;
+            } else {
+                // den = (1 + i h) - (i hm) gamma_{i-1}
+                const double2 den = (i == 0) ? cmk(1.0, h) : cmk(fma(hm, gprev.y, 1.0), fma(-hm, gprev.x, h));
+                r = crecip(den);
+            }
             const double2 rsub = cimul(hm, r);
             al = (i == 0) ? rsub : cneg(cmul(rsub, al));
+            if constexpr (ALPHA_LDS) {
+                if (i < M - 2) alds[(size_t)i * P * LANES] = al;
+            }
             const double2 d = (i == 0) ? cmul(r, rhs) : csub(cmul(r, rhs), cmul(rsub, dprev));
             const double2 g = cimul(hp, r);
-            sa = cadd(sa, cmul(pi, al));
-            sd = cadd(sd, cmul(pi, d));
-            if (i < M - 2) pi = cneg(cmul(pi, g));
+            if constexpr (!ALPHA_LDS) {
+                sa = cadd(sa, cmul(pi, al));
+                sd = cadd(sd, cmul(pi, d));
+                if (i < M - 2) pi = cneg(cmul(pi, g));
+            }
             ga[i] = g;
             de[i] = d;
             gprev = g;
@@ -342,7 +380,25 @@ __device__ __forceinline__ void cn_implicit_lowreg(double2 (&x)[M], const int k,
             suplast = hp;
         }
     }
-    const double2 gp0 = cmul(pi, gprev);  // pi_{m-2} gamma_{m-2}
+    double2 gp0;
+    if constexpr (ALPHA_LDS) {
+        // row 0 through (L_k, Y_k) by a backward sweep from row m-2 over the stored alpha_i
+        double2 ap0 = al, dp0 = dprev;
+        gp0 = gprev;
+#pragma unroll
+        for (int i = M - 3; i >= 0; --i) {
+            int off = i * P * LANES;
+            asm("" : "+v"(off));  // an opaque offset: read back from LDS, not forwarded from the store
+            const double2 a = alds[off];
+            ap0 = csub(a, cmul(ga[i], ap0));
+            gp0 = cneg(cmul(ga[i], gp0));
+            dp0 = csub(de[i], cmul(ga[i], dp0));
+        }
+        sa = ap0;
+        sd = dp0;
+    } else {
+        gp0 = cmul(pi, gprev);  // pi_{m-2} gamma_{m-2}
+    }
     __syncthreads();
     lds[k * LANES + ln] = sa;
     lds[(P + k) * LANES + ln] = gp0;
@@ -367,6 +423,18 @@ __device__ __forceinline__ void cn_implicit_lowreg(double2 (&x)[M], const int k,
     }
     double2 Lk, Yk;
     reduced_cr<LANES>(A, C, D, k, P, lds, Yk, Lk, ln);
+    if constexpr (ALPHA_LDS) {
+        // passes 2 and 3 in one: z_i = delta_i - alpha_i L_k - gamma_i z_{i+1}
+        double2 z = Yk;
+        x[M - 1] = z;
+#pragma unroll
+        for (int i = M - 2; i >= 0; --i) {
+            const double2 a = (i == M - 2) ? al : alds[(size_t)i * P * LANES];
+            z = csub(csub(de[i], cmul(a, Lk)), cmul(ga[i], z));
+            x[i] = z;
+        }
+        return;
+    }
     // pass 2: subtract alpha_i L_k
     double2 mu = cmk(0.0, 0.0);
 #pragma unroll

@@ -669,7 +669,7 @@ template <int M, int MAXT, int WPE>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_sh_rphase_vpair(ShSim* __restrict__ sims, const int2* __restrict__ pairs, int n_pairs,
                   const double* __restrict__ cl) {
-    extern __shared__ double2 lds[];  // 3 * P * 2: the solver's rows of both lines, interleaved
+    extern __shared__ double2 lds[];  // (3 + M - 2) * P * 2: the solver's rows and alpha_i of both lines, interleaved
     const int job = xcd_remap(blockIdx.x, gridDim.x);
     if (job >= n_pairs) return;
     const int2 pt = pairs[job];
@@ -698,7 +698,10 @@ k_sh_rphase_vpair(ShSim* __restrict__ sims, const int2* __restrict__ pairs, int 
     auto diag = [&](int i, double T) {
         const int j = k * M + i;
         const double r = fma((double)j, dr, 0.5 * dr);
-        const double U = (j < nr) ? cu * tdse_rcp(r * r) : 0.0;
+        // computed for padding rows too (r > 0): without the branch the two-line
+        // kernel's schedule fits its registers
+        const double Ur = cu * tdse_rcp(r * r);
+        const double U = (j < nr) ? Ur : 0.0;
         return sn * fma(ll1, U, T);
     };
     double carry = a_left;
@@ -720,7 +723,7 @@ k_sh_rphase_vpair(ShSim* __restrict__ sims, const int2* __restrict__ pairs, int 
     };
     double2 xl, xr;
     chunk_neighbours<M, 2>(x, k, P, lds, xl, xr, true, ln);
-    cn_implicit_lowreg<M, decltype(coef), decltype(coef2), true, 2>(x, k, P, coef, coef2, lds, xl, xr, ln);
+    cn_implicit_lowreg<M, decltype(coef), decltype(coef2), true, 2, NoRatio, true>(x, k, P, coef, coef2, lds, xl, xr, ln);
     if (fh != 0.0) sh_vpair_factors<M>(x, k, P, ln, lds, av, fp, true);
 #pragma unroll
     for (int i = 0; i < M; ++i) st2s(gl, (size_t)i * P, x[i]);
