@@ -539,7 +539,7 @@ void dispatch_vphase(tdse_ctx* ctx, const Bucket& b, int par, hipStream_t st) {
 
 template <int M, bool IMAG, int METHOD>
 void launch_line(tdse_ctx* ctx, const Bucket& b, int n_steps) {
-    const size_t lds = 3 * (size_t)b.P * sizeof(double2) + 64 * sizeof(double);
+    const size_t lds = (size_t)line_lds_rows(M, IMAG) * b.P * sizeof(double2) + 64 * sizeof(double);
     hipLaunchKernelGGL((k_line_steps<M, 512, IMAG, METHOD>), dim3((unsigned)b.lines.count), dim3(b.P), lds,
                        ctx->stream, ctx->d_line, b.lines.ptr, (int)b.lines.count, n_steps);
 }

@@ -163,6 +163,12 @@ __device__ __forceinline__ void line_observe(LineSim& S, const double2 (&x)[M], 
     }
 }
 
+// From 10 rows per thread the real-time solve keeps alpha_i in LDS (cn_implicit_lowreg's
+// ALPHA_LDS form): with pass 2 gone the deep tiles no longer spill registers.
+constexpr bool line_alpha_lds(int M) { return M >= 10; }
+// LDS rows of P double2 that the solver uses (the reduction scratch follows them)
+constexpr int line_lds_rows(int M, bool IMAG) { return (IMAG && line_alpha_lds(M)) ? 3 + M - 2 : 3; }
+
 // blockDim.x == P and m == M for every simulation launched together; one block per simulation.
 // IMAG: every w_n is purely imaginary (real time steps), which halves the
 // coefficient arithmetic; imaginary-time relaxation uses the general path.
@@ -171,7 +177,7 @@ __device__ __forceinline__ void line_observe(LineSim& S, const double2 (&x)[M], 
 template <int M, int MAXT, bool IMAG, int METHOD>
 __global__ void __launch_bounds__(MAXT) k_line_steps(LineSim* __restrict__ sims, const int2* __restrict__ ids,
                                                      int n_ids, int n_steps) {
-    extern __shared__ double2 lds[];  // 3 * P double2, then reduction scratch
+    extern __shared__ double2 lds[];  // the solver's line_lds_rows(M, IMAG) * P double2, then reduction scratch
     if ((int)blockIdx.x >= n_ids) return;
     LineSim& S = sims[ids[blockIdx.x].x];
     if (!S.active) return;
@@ -180,7 +186,7 @@ __global__ void __launch_bounds__(MAXT) k_line_steps(LineSim* __restrict__ sims,
     if (n > N) return;
     const int P = S.P, npad = S.npad;
     const int k = threadIdx.x;
-    double* red = reinterpret_cast<double*>(lds + 3 * P);
+    double* red = reinterpret_cast<double*>(lds + line_lds_rows(M, IMAG) * P);
     double2* __restrict__ g = S.g + k;
     const double* __restrict__ H = S.tab + LN_H * npad + k;
     const double* __restrict__ AP = S.tab + LN_AP * npad + k;
@@ -243,7 +249,7 @@ __global__ void __launch_bounds__(MAXT) k_line_steps(LineSim* __restrict__ sims,
                 return (j >= 1 && j + 1 < nz) ? 1.0 : 0.0;
             };
             const int ib = ((nz - 1) / M == k) ? (nz - 1) % M : -1;
-            cn_implicit_lowreg<M, decltype(coef), decltype(coef2), true, 1, decltype(ratio)>(
+            cn_implicit_lowreg<M, decltype(coef), decltype(coef2), true, 1, decltype(ratio), line_alpha_lds(M)>(
                 x, k, P, coef, coef2, lds, xl, xr, 0, ratio, ib);
             if constexpr (METHOD == LINE_SIO_LEN) line_phase<M>(x, k, sw * fs, S.zq0, S.dzq);
             if constexpr (METHOD == LINE_SIO_VEL) line_vel_pairs<M>(x, k, P, lds, sw * fs * S.vq, nz, true);
