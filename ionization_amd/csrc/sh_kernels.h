@@ -510,6 +510,9 @@ __device__ __forceinline__ void sh_vel_pair(double th, double2& s0, double2& s1,
     s1 = a1;
     d0 = b0;
     d1 = b1;
+    // the rotated rows are produced here, pair by pair: left to itself the compiler
+    // evaluates every pair's sine and cosine first and spills at 14 and 16 rows
+    asm("" : "+v"(s0.x), "+v"(s0.y), "+v"(s1.x), "+v"(s1.y), "+v"(d0.x), "+v"(d0.y), "+v"(d1.x), "+v"(d1.y));
 }
 
 template <int M>
