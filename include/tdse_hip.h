@@ -175,9 +175,12 @@ int tdse_line_add(tdse_ctx *ctx, const tdse_line_desc *desc, const double *table
  * evolution_methods.py:45-77): per step the explicit rho half, implicit z,
  * explicit z, implicit rho, then the mask, with the Coulomb potential and the
  * interaction split half-and-half between the directions
- * (mesh_operators.py:244-298). Device layout: g[rho][p], p < npz =
- * z_chunks * z_rows, position p holding z index (p % z_chunks) * z_rows +
- * p / z_chunks; rho padded to nrpad = rho_chunks * rho_rows.
+ * (mesh_operators.py:244-298). Device layout: rho rows r < nrpad =
+ * rho_chunks * rho_rows (even) by positions p < npz = z_chunks * z_rows,
+ * position p holding z index (p % z_chunks) * z_rows + p / z_chunks; the two
+ * rows of each pair (2 q, 2 q + 1) are interleaved position by position:
+ * element (r, p) at ((r / 2) * npz + p) * 2 + r % 2 (g0, basis, downloads and
+ * uploads alike).
  * The rho-sweep shape (rho_rows, lanes, rho_chunks) is one of (4, 4, <= 256),
  * (8, 4, <= 128), (4, 8, <= 128), (8, 8, <= 128), (16, 4, <= 128), with
  * rho_chunks * lanes a multiple of 64.

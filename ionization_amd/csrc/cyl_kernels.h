@@ -78,6 +78,14 @@ __device__ __forceinline__ double cyl_half_potential(const CylSim& S, double z, 
 
 // Off-diagonal of H_rho between rho indices j and j + 1: rp c(j + 1),
 // c(m) = m / sqrt(m^2 - 1/4) (mesh_operators.py:582-592); 0 past the line.
+// Device order of a cylindrical g (and of its basis states): rho rows in pairs
+// (2 q, 2 q + 1) interleaved position by position, element (r, p) at
+// ((r / 2) npz + p) 2 + r % 2 (nrpad is even). The z-sweep's threads then read
+// whole lines in pairs and the rho-sweep's four positions whole 128 B lines.
+__device__ __forceinline__ size_t cyl_at(int r, int p, int npz) {
+    return ((size_t)(r >> 1) * npz + p) * 2 + (r & 1);
+}
+
 __device__ __forceinline__ double cyl_rho_off(const CylSim& S, int j) {
     if (j < 0 || j + 1 >= S.nr) return 0.0;
     const double m = (double)(j + 1);
@@ -102,33 +110,37 @@ __device__ __forceinline__ double cyl_mask_r(const CylSim& S, double r) {
 
 // ------------------------------------------------------------------ z-sweep
 //
-// One workgroup of Pz threads per (simulation, rho-line); Mz rows per thread.
+// One workgroup of 2 Pz threads per (simulation, pair of rho-lines 2 q, 2 q + 1);
+// Mz rows per thread. Thread t works on line ln = t & 1 and chunk k = t >> 1:
+// the two lines of a pair are interleaved in memory (cyl_at), so neighbouring
+// threads read neighbouring 16 B and a wave reads 1 KiB contiguous.
 // (1 + i tau H_z) y = (1 - i tau H_z) x with H_z tridiagonal: diagonal
 // -2 zp + V/2 - z q E / 2, off-diagonals zp inside the line
-// (mesh_operators.py:561-575). FORM as for k_sh_rphase.
+// (mesh_operators.py:561-575); the launch uses the explicit form. A pair may end in a
+// padding line (r = nr for odd nr): it holds zeros and stays zero.
 template <int M, int MAXT, int WPE, int FORM>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
-k_cyl_z(CylSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines) {
-    extern __shared__ double2 lds[];  // 3 * Pz, then M * Pz for CN_CAYLEY_LDS
-    const int line = blockIdx.x;
-    if (line >= n_lines) return;
-    const int2 lt = lines[line];
+k_cyl_z(CylSim* __restrict__ sims, const int2* __restrict__ pairs, int n_pairs) {
+    extern __shared__ double2 lds[];  // 3 * Pz * 2, then M * Pz * 2 for CN_CAYLEY_LDS
+    const int job = blockIdx.x;
+    if (job >= n_pairs) return;
+    const int2 lt = pairs[job];
     CylSim& S = sims[lt.x];
     const int n = S.step;
     if (!S.active || n >= S.N) return;  // uniform across the block
     const int P = S.Pz, nz = S.nz;
-    const int k = threadIdx.x;
+    const int ln = threadIdx.x & 1, k = threadIdx.x >> 1;
     const double tau = ld1(S.ts, n);
     const double E = ld1(S.fs, n);
-    const double rho = fma((double)lt.y, S.drho, S.rho0);
+    const double rho = fma((double)(2 * lt.y + ln), S.drho, S.rho0);
     const double tzp = tau * S.zp;
     const double tdiag = tau * (S.zp * -2.0);
-    double2* __restrict__ gl = S.g + (size_t)lt.y * S.npz + k;
+    double2* __restrict__ gl = S.g + cyl_at(2 * lt.y + ln, k, S.npz);
     const double z0 = S.z0, dz = S.dz;
 
     double2 x[M];
 #pragma unroll
-    for (int i = 0; i < M; ++i) x[i] = ld2(gl, (size_t)i * P);
+    for (int i = 0; i < M; ++i) x[i] = ld2(gl, (size_t)i * P * 2);
     // coefficients from the grid, no table reads
     auto coef = [&](int i, double& hm, double& h, double& hp) {
         const int j = k * M + i;
@@ -149,7 +161,8 @@ k_cyl_z(CylSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines) 
         return (j >= 1 && j + 1 < nz) ? 1.0 : 0.0;
     };
     const int ib = ((nz - 1) / M == k) ? (nz - 1) % M : -1;
-    cn_line_finish<M, FORM>(x, gl, k, P, coef, coef2, lds, ratio, ib);
+    cn_line_finish<M, FORM, decltype(coef), decltype(coef2), decltype(ratio), 2>(x, gl, k, P, coef, coef2, lds, ratio,
+                                                                               ib, ln, 2 * P);
 }
 
 // ---------------------------------------------------------------- rho-sweep
@@ -196,12 +209,14 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 
     const int jz = (p % S.Pz) * S.Mz + p / S.Pz;
     const bool line_live = jz < S.nz;
     const double z = ld1(S.zt, p);
-    double2* __restrict__ gl = S.g + (size_t)(c * M) * npz + p;
+    // rows c M .. c M + M - 1 (M even): row c M + i at gl[cyl_at(i, 0, npz)]; with the two
+    // rows of a pair adjacent, LANES = 4 neighbouring positions read whole 128 B lines
+    double2* __restrict__ gl = S.g + cyl_at(c * M, p, npz);
     double* red = reinterpret_cast<double*>(lds + 3 * P * LANES);
 
     double2 x[M];
 #pragma unroll
-    for (int i = 0; i < M; ++i) x[i] = ld2(gl, (size_t)i * npz);
+    for (int i = 0; i < M; ++i) x[i] = ld2(gl, cyl_at(i, 0, npz));
 
     // rho-direction coefficients of row i (rho index j = c M + i) for (tau, E),
     // from the grid: rho_j recomputed, the sub-diagonal carried from the
@@ -287,11 +302,11 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 
         const double tot = cyl_block_sum<LANES>(acc, red, t, blockDim.x);
         if (t == 0) part[0] = tot;
         for (int b = 0; b < nb; ++b) {
-            const double2* phi = S.basis + (size_t)b * S.nrpad * npz + (size_t)(c * M) * npz + p;
+            const double2* phi = S.basis + (size_t)b * S.nrpad * npz + cyl_at(c * M, p, npz);
             double re = 0.0, im = 0.0;
 #pragma unroll
             for (int i = 0; i < M; ++i) {
-                const double2 f = ld2(phi, (size_t)i * npz);  // conj(f) x
+                const double2 f = ld2(phi, cyl_at(i, 0, npz));  // conj(f) x
                 re += fma(f.x, x[i].x, f.y * x[i].y);
                 im += fma(f.x, x[i].y, -f.y * x[i].x);
             }
@@ -324,7 +339,7 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 
         }
     }
 #pragma unroll
-    for (int i = 0; i < M; ++i) st2(gl, (size_t)i * npz, x[i]);
+    for (int i = 0; i < M; ++i) st2(gl, cyl_at(i, 0, npz), x[i]);
 }
 
 // Reduce the rho-sweep partials of every simulation at a data time: job

@@ -329,7 +329,7 @@ int rebuild_tables(tdse_ctx* ctx) {
         cjobs.push_back(make_int2(s, -1));
         for (int b = 0; b < C.n_basis; ++b) cjobs.push_back(make_int2(s, b));
         auto& vz = czb[{C.Mz, C.Pz}];
-        for (int r = 0; r < C.nr; ++r) vz.push_back(make_int2(s, r));
+        for (int q = 0; 2 * q < C.nr; ++q) vz.push_back(make_int2(s, q));  // pairs of rho-lines
         auto& vr = crb[{C.Mr + 64 * C.lanes, C.Cr}];
         for (int g = 0; g < C.n_groups; ++g) vr.push_back(make_int2(s, g));
     }
@@ -567,32 +567,26 @@ void dispatch_line(tdse_ctx* ctx, const Bucket& b, int n_steps) {
     else dispatch_line_m<true, LINE_ADI>(ctx, b, n_steps);
 }
 
+// The z-sweep solves two rho-lines per workgroup (2 Pz threads), explicit form
+// (with two lines per workgroup the Cayley forms measured slower: re-reading the
+// rows 0.44 ms, keeping them in LDS 0.64 ms per sweep against 0.40 ms).
 template <int M, int MAXT, int WPE>
 void launch_cyl_z(tdse_ctx* ctx, const Bucket& b) {
-    const int form = cn_form(M, b.P);
-    const size_t lds = cn_lds_bytes(M, b.P, form);
-    const dim3 grid((unsigned)b.lines.count), block(b.P);
-    if (form == CN_EXPLICIT)
-        hipLaunchKernelGGL((k_cyl_z<M, MAXT, WPE, (M >= 10 ? CN_EXPLICIT : CN_CAYLEY_MEM)>), grid, block, lds,
-                           ctx->stream, ctx->d_cyl, b.lines.ptr, (int)b.lines.count);
-    else if (form == CN_CAYLEY_LDS)
-        hipLaunchKernelGGL((k_cyl_z<M, MAXT, WPE, (M >= 10 ? CN_EXPLICIT : CN_CAYLEY_LDS)>), grid, block, lds,
-                           ctx->stream, ctx->d_cyl, b.lines.ptr, (int)b.lines.count);
-    else
-        hipLaunchKernelGGL((k_cyl_z<M, MAXT, WPE, (M >= 10 ? CN_EXPLICIT : CN_CAYLEY_MEM)>), grid, block, lds,
-                           ctx->stream, ctx->d_cyl, b.lines.ptr, (int)b.lines.count);
+    const size_t lds = 2 * cn_lds_bytes(M, b.P, CN_EXPLICIT);
+    hipLaunchKernelGGL((k_cyl_z<M, MAXT, WPE, CN_EXPLICIT>), dim3((unsigned)b.lines.count), dim3(2 * b.P), lds,
+                       ctx->stream, ctx->d_cyl, b.lines.ptr, (int)b.lines.count);
 }
 
 void dispatch_cyl_z(tdse_ctx* ctx, const Bucket& b) {
     switch (b.M) {
-        case 2: launch_cyl_z<2, 512, 4>(ctx, b); break;
-        case 4: launch_cyl_z<4, 512, 4>(ctx, b); break;
-        case 6: launch_cyl_z<6, 512, 4>(ctx, b); break;
-        case 8: launch_cyl_z<8, 512, 4>(ctx, b); break;
-        case 10: launch_cyl_z<10, 256, 3>(ctx, b); break;
-        case 12: launch_cyl_z<12, 256, 2>(ctx, b); break;
-        case 14: launch_cyl_z<14, 256, 2>(ctx, b); break;
-        default: launch_cyl_z<16, 256, 1>(ctx, b); break;
+        case 2: launch_cyl_z<2, 1024, 4>(ctx, b); break;
+        case 4: launch_cyl_z<4, 1024, 4>(ctx, b); break;
+        case 6: launch_cyl_z<6, 1024, 4>(ctx, b); break;
+        case 8: launch_cyl_z<8, 1024, 4>(ctx, b); break;
+        case 10: launch_cyl_z<10, 512, 3>(ctx, b); break;
+        case 12: launch_cyl_z<12, 512, 2>(ctx, b); break;
+        case 14: launch_cyl_z<14, 512, 2>(ctx, b); break;
+        default: launch_cyl_z<16, 512, 2>(ctx, b); break;
     }
 }
 

@@ -373,30 +373,33 @@ __global__ void __launch_bounds__(256) k_sh_observe(ShSim* __restrict__ sims, co
 enum { CN_CAYLEY_MEM = 0, CN_CAYLEY_LDS = 1, CN_EXPLICIT = 2 };
 
 // Shared tail of the CN line kernels: x holds the thread's M rows of the line
-// at gl (stride P); solves and stores y = (1 + i H~)^-1 (1 - i H~) x.
-// Ratio / ib: optional pass-2 pivot ratios (cn_implicit_lowreg).
-template <int M, int FORM, class Coef, class Coef2, class Ratio = NoRatio>
+// at gl (row i at gl[i * stride]; stride P by default); solves and stores
+// y = (1 + i H~)^-1 (1 - i H~) x. Ratio / ib: optional pass-2 pivot ratios
+// (cn_implicit_lowreg). LANES / ln: LANES lines per workgroup, interleaved in LDS.
+template <int M, int FORM, class Coef, class Coef2, class Ratio = NoRatio, int LANES = 1>
 __device__ __forceinline__ void cn_line_finish(double2 (&x)[M], double2* __restrict__ gl, const int k, const int P,
                                                Coef coef, Coef2 coef2, double2* __restrict__ lds,
-                                               Ratio ratio = Ratio{}, const int ib = -1) {
+                                               Ratio ratio = Ratio{}, const int ib = -1, const int ln = 0,
+                                               const int stride = -1) {
+    const size_t st = stride < 0 ? (size_t)P : (size_t)stride;
     if constexpr (FORM == CN_EXPLICIT) {
         double2 xl, xr;
-        chunk_neighbours<M>(x, k, P, lds, xl, xr, false);
-        cn_implicit_lowreg<M, Coef, Coef2, true, 1, Ratio>(x, k, P, coef, coef2, lds, xl, xr, 0, ratio, ib);
+        chunk_neighbours<M, LANES>(x, k, P, lds, xl, xr, false, ln);
+        cn_implicit_lowreg<M, Coef, Coef2, true, LANES, Ratio>(x, k, P, coef, coef2, lds, xl, xr, ln, ratio, ib);
 #pragma unroll
-        for (int i = 0; i < M; ++i) st2s(gl, (size_t)i * P, x[i]);
+        for (int i = 0; i < M; ++i) st2s(gl, (size_t)i * st, x[i]);
     } else {
-        double2* keep = lds + 3 * P;  // CN_CAYLEY_LDS: this thread's own rows only
+        double2* keep = lds + 3 * P * LANES;  // CN_CAYLEY_LDS: this thread's own rows only
         if constexpr (FORM == CN_CAYLEY_LDS) {
 #pragma unroll
-            for (int i = 0; i < M; ++i) keep[i * P + k] = x[i];
+            for (int i = 0; i < M; ++i) keep[(i * P + k) * LANES + ln] = x[i];
         }
-        cn_implicit_lowreg<M, Coef, Coef2, false, 1, Ratio>(x, k, P, coef, coef2, lds, cmk(0.0, 0.0), cmk(0.0, 0.0),
-                                                            0, ratio, ib);
+        cn_implicit_lowreg<M, Coef, Coef2, false, LANES, Ratio>(x, k, P, coef, coef2, lds, cmk(0.0, 0.0),
+                                                                cmk(0.0, 0.0), ln, ratio, ib);
 #pragma unroll
         for (int i = 0; i < M; ++i) {
-            const double2 x0 = (FORM == CN_CAYLEY_LDS) ? keep[i * P + k] : ld2(gl, (size_t)i * P);
-            st2s(gl, (size_t)i * P, cmk(fma(2.0, x[i].x, -x0.x), fma(2.0, x[i].y, -x0.y)));
+            const double2 x0 = (FORM == CN_CAYLEY_LDS) ? keep[(i * P + k) * LANES + ln] : ld2(gl, (size_t)i * st);
+            st2s(gl, (size_t)i * st, cmk(fma(2.0, x[i].x, -x0.x), fma(2.0, x[i].y, -x0.y)));
         }
     }
 }

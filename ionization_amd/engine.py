@@ -480,22 +480,34 @@ def choose_cyl_layout(z_points: int, rho_points: int, z_rows: Optional[int] = No
 
 
 def to_cyl_positions(g: np.ndarray, Pz: int, Mz: int, nrpad: int) -> np.ndarray:
-    """(..., nz, nr) natural order -> (..., nrpad, Pz Mz) device order, zero padding."""
+    """
+    (..., nz, nr) natural order -> the device order, zero padding: rho rows r < nrpad
+    (even) by z positions p < Pz Mz (position p holds z index (p % Pz) Mz + p // Pz), with
+    the two rows of each pair (2 q, 2 q + 1) interleaved position by position: element
+    (r, p) at flat index ((r // 2) npz + p) 2 + r % 2 of the trailing (nrpad, npz) block.
+    """
     g = np.asarray(g)
     nz, nr = g.shape[-2:]
+    if nrpad % 2:
+        raise DeviceError("the padded rho rows of a cylindrical mesh must be even")
+    npz = Pz * Mz
     jz = position_index(nz, Pz, Mz)
     live = jz < nz
-    out = np.zeros(g.shape[:-2] + (nrpad, Pz * Mz), dtype=g.dtype)
-    out[..., :nr, live] = np.swapaxes(g[..., jz[live], :], -1, -2)
-    return out
+    rows = np.zeros(g.shape[:-2] + (nrpad, npz), dtype=g.dtype)
+    rows[..., :nr, live] = np.swapaxes(g[..., jz[live], :], -1, -2)
+    pairs = rows.reshape(g.shape[:-2] + (nrpad // 2, 2, npz))
+    return np.ascontiguousarray(np.swapaxes(pairs, -1, -2)).reshape(g.shape[:-2] + (nrpad, npz))
 
 
 def from_cyl_positions(a: np.ndarray, nz: int, nr: int, Pz: int, Mz: int) -> np.ndarray:
     """Inverse of to_cyl_positions."""
+    a = np.asarray(a)
+    nrpad, npz = a.shape[-2:]
+    rows = np.swapaxes(a.reshape(a.shape[:-2] + (nrpad // 2, npz, 2)), -1, -2).reshape(a.shape[:-2] + (nrpad, npz))
     jz = position_index(nz, Pz, Mz)
     live = jz < nz
     out = np.empty(a.shape[:-2] + (nz, nr), dtype=a.dtype)
-    out[..., jz[live], :] = np.swapaxes(a[..., :nr, live], -1, -2)
+    out[..., jz[live], :] = np.swapaxes(rows[..., :nr, live], -1, -2)
     return out
 
 

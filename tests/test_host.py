@@ -219,6 +219,11 @@ def test_cyl_layout_covers_mesh_and_round_trips(nz, nr):
     assert d.shape == (Cr * Mr, Pz * Mz)
     assert np.count_nonzero(d) == nz * nr  # padding is zero
     assert np.array_equal(engine.from_cyl_positions(d, nz, nr, Pz, Mz), g)
+    # the device order the kernels index (cyl_at): rho rows in interleaved pairs
+    flat, npz = d.reshape(-1), Pz * Mz
+    for j, r in [(0, 0), (nz - 1, nr - 1), (nz // 2, 1), (1, nr // 2), (nz - 2, nr - 2)]:
+        p = (j % Mz) * Pz + j // Mz  # position of z index j
+        assert flat[((r // 2) * npz + p) * 2 + r % 2] == g[j, r]
 
 
 def test_cyl_layout_forced_shapes_and_rejects_unknown():
