@@ -455,44 +455,44 @@ void dispatch_rphase(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
 }
 
 template <int M, int MAXT, int WPE, bool EXPL>
-void launch_adi_r(tdse_ctx* ctx, const Bucket& b) {
+void launch_adi_r(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
     const size_t lds = 3 * (size_t)b.P * sizeof(double2);
-    hipLaunchKernelGGL((k_sh_adi_r<M, MAXT, WPE, EXPL>), dim3((unsigned)b.lines.count), dim3(b.P), lds, ctx->stream,
+    hipLaunchKernelGGL((k_sh_adi_r<M, MAXT, WPE, EXPL>), dim3((unsigned)b.lines.count), dim3(b.P), lds, st,
                        ctx->d_sh, b.lines.ptr, (int)b.lines.count);
 }
 
 // the R-phase's launch shapes
 template <bool EXPL>
-void dispatch_adi_r(tdse_ctx* ctx, const Bucket& b) {
+void dispatch_adi_r(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
     switch (b.M) {
-        case 2: launch_adi_r<2, 512, 4, EXPL>(ctx, b); break;
-        case 4: launch_adi_r<4, 512, 4, EXPL>(ctx, b); break;
-        case 6: launch_adi_r<6, 512, 4, EXPL>(ctx, b); break;
-        case 8: launch_adi_r<8, 512, 4, EXPL>(ctx, b); break;
-        case 10: b.P > 256 ? launch_adi_r<10, 512, 1, EXPL>(ctx, b) : launch_adi_r<10, 256, 3, EXPL>(ctx, b); break;
-        case 12: b.P > 256 ? launch_adi_r<12, 512, 1, EXPL>(ctx, b) : launch_adi_r<12, 256, 2, EXPL>(ctx, b); break;
-        case 14: b.P > 256 ? launch_adi_r<14, 512, 1, EXPL>(ctx, b) : launch_adi_r<14, 256, 2, EXPL>(ctx, b); break;
-        default: b.P > 256 ? launch_adi_r<16, 512, 1, EXPL>(ctx, b) : launch_adi_r<16, 256, 1, EXPL>(ctx, b); break;
+        case 2: launch_adi_r<2, 512, 4, EXPL>(ctx, b, st); break;
+        case 4: launch_adi_r<4, 512, 4, EXPL>(ctx, b, st); break;
+        case 6: launch_adi_r<6, 512, 4, EXPL>(ctx, b, st); break;
+        case 8: launch_adi_r<8, 512, 4, EXPL>(ctx, b, st); break;
+        case 10: b.P > 256 ? launch_adi_r<10, 512, 1, EXPL>(ctx, b, st) : launch_adi_r<10, 256, 3, EXPL>(ctx, b, st); break;
+        case 12: b.P > 256 ? launch_adi_r<12, 512, 1, EXPL>(ctx, b, st) : launch_adi_r<12, 256, 2, EXPL>(ctx, b, st); break;
+        case 14: b.P > 256 ? launch_adi_r<14, 512, 1, EXPL>(ctx, b, st) : launch_adi_r<14, 256, 2, EXPL>(ctx, b, st); break;
+        default: b.P > 256 ? launch_adi_r<16, 512, 1, EXPL>(ctx, b, st) : launch_adi_r<16, 256, 1, EXPL>(ctx, b, st); break;
     }
 }
 
 template <int M>
-void launch_adi_lp(tdse_ctx* ctx, const Bucket& b) {
+void launch_adi_lp(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
     const size_t lds = 3 * (size_t)kAdiChunks * kAdiLanes * sizeof(double2);
-    hipLaunchKernelGGL(k_sh_adi_lp<M>, dim3((unsigned)b.lines.count), dim3(kAdiChunks * kAdiLanes), lds, ctx->stream,
+    hipLaunchKernelGGL(k_sh_adi_lp<M>, dim3((unsigned)b.lines.count), dim3(kAdiChunks * kAdiLanes), lds, st,
                        ctx->d_sh, b.lines.ptr, (int)b.lines.count, ctx->d_cl);
 }
 
-void dispatch_adi_lp(tdse_ctx* ctx, const Bucket& b) {
+void dispatch_adi_lp(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
     switch (b.M) {
-        case 2: launch_adi_lp<2>(ctx, b); break;
-        case 4: launch_adi_lp<4>(ctx, b); break;
-        case 6: launch_adi_lp<6>(ctx, b); break;
-        case 8: launch_adi_lp<8>(ctx, b); break;
-        case 10: launch_adi_lp<10>(ctx, b); break;
-        case 12: launch_adi_lp<12>(ctx, b); break;
-        case 14: launch_adi_lp<14>(ctx, b); break;
-        default: launch_adi_lp<16>(ctx, b); break;
+        case 2: launch_adi_lp<2>(ctx, b, st); break;
+        case 4: launch_adi_lp<4>(ctx, b, st); break;
+        case 6: launch_adi_lp<6>(ctx, b, st); break;
+        case 8: launch_adi_lp<8>(ctx, b, st); break;
+        case 10: launch_adi_lp<10>(ctx, b, st); break;
+        case 12: launch_adi_lp<12>(ctx, b, st); break;
+        case 14: launch_adi_lp<14>(ctx, b, st); break;
+        default: launch_adi_lp<16>(ctx, b, st); break;
     }
 }
 
@@ -678,15 +678,39 @@ int launch_step(tdse_ctx* ctx, bool cyl_pre) {
                 HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join[j], 0));
             }
         }
-        // AlternatingDirectionImplicit simulations: explicit H0, the l-direction factor, implicit H0
+        // AlternatingDirectionImplicit simulations: explicit H0, the l-direction factor, implicit H0.
+        // Within each of the three sweeps the buckets are independent: they run concurrently
+        // over the context stream and the side streams (fork and join by events), and each
+        // sweep starts after the whole previous one.
+        auto fan = [&](std::vector<Bucket>& bs, auto&& launch) -> int {
+            if (ctx->profiling || bs.size() < 2) {
+                for (auto& b : bs) launch(b, ctx->stream);
+                return TDSE_OK;
+            }
+            HIP_TRY(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
+            for (int j = 0; j < kSideStreams; ++j) HIP_TRY(ctx, hipStreamWaitEvent(ctx->side[j], ctx->ev_fork, 0));
+            for (size_t i = 0; i < bs.size(); ++i) {
+                const int j = (int)(i % (kSideStreams + 1));
+                launch(bs[i], j == 0 ? ctx->stream : ctx->side[j - 1]);
+            }
+            for (int j = 0; j < kSideStreams; ++j) {
+                HIP_TRY(ctx, hipEventRecord(ctx->ev_join[j], ctx->side[j]));
+                HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join[j], 0));
+            }
+            return TDSE_OK;
+        };
         if (!ctx->sh_adi_buckets.empty()) {
-            for (auto& b : ctx->sh_adi_buckets) dispatch_adi_r<true>(ctx, b);
-            for (auto& b : ctx->sh_adi_lbuckets) dispatch_adi_lp(ctx, b);
+            int rc;
+            if ((rc = fan(ctx->sh_adi_buckets, [&](const Bucket& b, hipStream_t st) { dispatch_adi_r<true>(ctx, b, st); })))
+                return rc;
+            if ((rc = fan(ctx->sh_adi_lbuckets, [&](const Bucket& b, hipStream_t st) { dispatch_adi_lp(ctx, b, st); })))
+                return rc;
             const int na = (int)ctx->adi_waves.count;
             if (na > 0)
                 hipLaunchKernelGGL(k_sh_adi_l, dim3((na * TDSE_WAVE + 255) / 256), dim3(256), 0, ctx->stream,
                                    ctx->d_sh, ctx->adi_waves.ptr, na, ctx->d_cl);
-            for (auto& b : ctx->sh_adi_buckets) dispatch_adi_r<false>(ctx, b);
+            if ((rc = fan(ctx->sh_adi_buckets, [&](const Bucket& b, hipStream_t st) { dispatch_adi_r<false>(ctx, b, st); })))
+                return rc;
         }
         const int ns = (int)ctx->sh.size();
         hipLaunchKernelGGL(k_sh_advance, dim3((ns + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh, ns);
