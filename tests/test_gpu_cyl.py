@@ -67,6 +67,20 @@ def test_cyl_adi_matches_oracle_z_rows(device_ctx, z_rows):
     _check(case, *_run_device(device_ctx, c))
 
 
+@pytest.mark.parametrize("z_points, rho_points, z_rows", [(150, 41, None), (120, 45, 16), (4000, 21, 8)])
+def test_cyl_odd_rho_rows_and_widest_z_sweep(device_ctx, z_points, rho_points, z_rows):
+    """The rho rows are stored in interleaved pairs: an odd rho count ends in a pair with a
+    padding row (solved by the z-sweep as zeros); 4000 z points with 8 rows per thread give
+    512 z chunks, the z-sweep's widest workgroup (two lines, 1024 threads)."""
+    case = cyl_case(z_bound_bohr=40.0 if z_points > 1000 else 20.0, z_points=z_points, rho_points=rho_points,
+                    z_rows=z_rows, t_half_as=6.0 if z_points > 1000 else 15.0, store_every=3)
+    c = case["compiled"]
+    print(f"Cyl {z_points} x {rho_points}: Pz={c.Pz} Mz={c.Mz} Cr={c.Cr} Mr={c.Mr} nrpad={c.nrpad}", flush=True)
+    if z_points > 1000:
+        assert (c.Pz, c.Mz) == (512, 8)
+    _check(case, *_run_device(device_ctx, c))
+
+
 def test_cyl_hold_and_per_step_match_fused(device_ctx):
     """Exact-state stepping (one step per run, rho explicit half deferred) agrees with the fused run."""
     comp = cyl_case(t_half_as=10.0)["compiled"]
