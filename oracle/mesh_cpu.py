@@ -447,13 +447,15 @@ class SHVelocityOracle(SHOracle):
             g = g * self.mask_mesh
         return g
 
-    def run(self, g0, times, nominal_time_step=None, store_every=1, basis=(), by_l=False):
+    def run(self, g0, times, nominal_time_step=None, store_every=1, basis=(), by_l=False, start=0):
         """MeshSimulation.run with the velocity-gauge step; times_to_current = times[: i + 2]
-        when evolving from index i (sims.py:195-196, 317-321)."""
+        when evolving from index i (sims.py:195-196, 317-321). start > 0: g0 is the state at
+        times[start] (a resumed run, sims.py:255-270); the vector potential still integrates the
+        field from times[0], and data are taken from index start on."""
         g = np.array(g0, dtype=np.complex128)
         didx = set(data_indices(len(times), store_every).tolist())
         norms, ips, lnorm, nbl = [], [], [], []
-        for i in range(len(times)):
+        for i in range(start, len(times)):
             if i in didx:
                 norms.append(self.norm(g))
                 ips.append([self.inner_product(phi, l, g) for (phi, l) in basis])

@@ -8,6 +8,9 @@ Every workload at its real size on the device, against the oracle wherever the o
              three draws (l_bound 283 and 143, which are odd, and 138; 2066-3874 radial points;
              3.3-7.8 J/cm^2) checked against the oracle over the 100 steps around their pulse
              peaks, starting from the device's own state there.
+             The same 256 draws in the velocity gauge and with AlternatingDirectionImplicit, in
+             one batch each, every draw started from its initial state shortly before its peak:
+             two draws per method checked against the oracle across the peak (10-40 steps).
   cyl_scan   all 32 Gaussian-pulse simulations of 2000 x 1000 points with their mask, every one to
              its last step; the strongest-field one checked against the oracle over 16 steps around
              its pulse peak.
@@ -23,7 +26,7 @@ import os
 import numpy as np
 import pytest
 
-from ionization_amd import engine, serve, workloads
+from ionization_amd import engine, mesh, serve, workloads
 from ionization_amd import units as u
 from oracle import mesh_cpu, physics
 from tests.conftest import rel_l2
@@ -147,6 +150,62 @@ def test_sh_scan_all_256_draws_in_one_batch():
         assert err <= 1e-10
         assert abs(nrm(g_end) - nrm(g_ref)) <= 1e-9 and abs(ov(g_end) - ov(g_ref)) <= 1e-9
         assert rel_l2(g_end, g_start) > 1e-3  # the window does something
+
+
+# the other two SphericalHarmonicMesh methods on the same 256 draws: (draw, steps) checked
+# against the oracle from the draw's numeric ground state uploaded just before its pulse peak
+# (the oracle's velocity step at 283 x 3646 takes about 2 s, so its windows are short)
+SCAN_OTHER_CHECKED = {"velocity": ((195, 24), (239, 10)), "adi": ((29, 40), (239, 40))}
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("method", ["velocity", "adi"])
+def test_sh_scan_velocity_gauge_and_adi_all_256_draws_in_one_batch(method):
+    params = workloads.sh_scan_params(256, 0)
+    kw = (dict(operators=mesh.SphericalHarmonicVelocityGaugeOperators()) if method == "velocity"
+          else dict(evolution_method=mesh.AlternatingDirectionImplicit()))
+    sims = [workloads.sh_scan_spec(p, **kw).to_sim() for p in params]
+    comps = [sim.spec.evolution_method.compile(sim) for sim in sims]
+    checked = dict(SCAN_OTHER_CHECKED[method])
+    steps = max(checked.values())
+    # every draw starts from its initial state at step (peak - steps), so the window crosses the peak
+    starts = [c.n_steps // 2 - steps for c in comps]
+    pool = int(sum(c.footprint_bytes for c in comps) * 1.1) + (256 << 20)
+    print(f"sh_scan {method}: 256 draws, {sum(c.mesh_points for c in comps) / 1e6:.1f} M points", flush=True)
+    got = {}
+    with engine.DeviceContext(pool_bytes=pool) as ctx:
+        slots = [ctx.add_sh(c) for c in comps]
+        for c, sl, st in zip(comps, slots, starts):
+            ctx.upload_state(engine.KIND_SH, sl, engine.from_positions(c.g0, c.r_points, c.P, c.m), st)
+        done = 0
+        for k, n in sorted(checked.items(), key=lambda kv: kv[1]):
+            ctx.run(n - done)
+            done = n
+            ctx.materialize(engine.KIND_SH, slots[k])
+            assert ctx.state(engine.KIND_SH, slots[k])[0] == starts[k] + n
+            got[k] = ctx.download_sh(slots[k]).g
+        ctx.run(steps + 5 - done)  # every draw past its peak, all still finite
+        for i, sl in enumerate(slots):
+            r = ctx.download_sh(sl)
+            assert np.all(np.isfinite(r.g)), i
+    for k, n in checked.items():
+        p, c, sim = params[k], comps[k], sims[k]
+        g0 = engine.from_positions(c.g0, c.r_points, c.P, c.m)
+        orc = (mesh_cpu.SHVelocityOracle if method == "velocity" else mesh_cpu.SHOracle)(
+            r_bound=p["r_bound"], r_points=p["r_points"], l_bound=p["l_bound"], potential=physics.coulomb,
+            field=lambda t: physics.sinc_pulse_field(t, pulse_width=p["pulse_width"], fluence=p["fluence"],
+                                                     phase=p["phase"]),
+            mask=lambda r: physics.radial_cosine_mask(r, 0.9 * p["r_bound"], p["r_bound"]))
+        s0 = starts[k]
+        print(f"sh_scan {method} draw {k}: {p['l_bound']} x {p['r_points']}, oracle steps {s0}..{s0 + n}", flush=True)
+        if method == "adi":
+            g_ref = orc.run(g0, sim.times[s0:s0 + n + 1], sim.spec.time_step, 100, adi=True)[0]
+        else:  # A(t) integrates the field from the first time of the run
+            g_ref = orc.run(g0, sim.times[:s0 + n + 1], sim.spec.time_step, 100, start=s0)[0]
+        err = rel_l2(got[k], g_ref)
+        print(f"  rel L2 {err:.2e}; the window moved g by {rel_l2(got[k], g0):.2e}", flush=True)
+        assert err <= 1e-10
+        assert rel_l2(got[k], g0) > 1e-6  # the window does something
 
 
 CYL_STEPS = 16

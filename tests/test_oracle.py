@@ -636,6 +636,25 @@ def test_velocity_gauge_evolution_without_mask_is_unitary():
     assert abs(norms[-1] - norms[0]) < 1e-12 * norms[0]
 
 
+def test_velocity_oracle_resumed_run_equals_the_uninterrupted_one():
+    """run(start=k) continues from the state at times[k] with the vector potential still
+    integrated from times[0]: stopping after k steps and resuming gives the same g, and a
+    resume that integrated A from times[k] instead would not."""
+    fld = lambda t: physics.sinc_pulse_field(t, pulse_width=50 * u.asec, fluence=5 * u.Jcm2)
+    o = mesh_cpu.SHVelocityOracle(r_bound=20 * u.bohr_radius, r_points=40, l_bound=6, potential=physics.coulomb,
+                                  field=fld, mask=lambda r: physics.radial_cosine_mask(r, 15 * u.bohr_radius,
+                                                                                       20 * u.bohr_radius))
+    rng = np.random.default_rng(11)
+    g = rng.standard_normal((6, 40)) + 1j * rng.standard_normal((6, 40))
+    times = mesh_cpu.time_grid(-10 * u.asec, 10 * u.asec, 1 * u.asec)
+    k = 9
+    g_full = o.run(g, times)[0]
+    g_mid = o.run(g, times[:k + 1])[0]
+    assert np.array_equal(o.run(g_mid, times, start=k)[0], g_full)
+    g_wrong = o.run(g_mid, times[k:])[0]
+    assert np.linalg.norm(g_wrong - g_full) > 1e-6 * np.linalg.norm(g_full)
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 8, 11, 30])
 def test_simps_old_matches_scipy_simpson_on_odd_counts_and_avg_rule_on_even(n):
     """
