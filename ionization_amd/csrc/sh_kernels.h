@@ -154,10 +154,15 @@ __global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, con
     const bool do_start = n < N && !end_only;
     const bool observe = !end_only && sh_is_data_step(n, N, S.store_every);
     // Every wave writes its rows to the other buffer, even with nothing to apply (a copy).
-    const bool fuse_e = do_end && do_start && !observe;
+    const bool both = do_end && do_start;
 
     const int L = S.L, P = S.P, m = S.m, npad = S.npad;
     const int lo = wt.lo, hi = wt.hi;
+    // read once: inside the row loop the snapshot stores could otherwise make the compiler
+    // reload these fields every iteration
+    const int LB = S.LB;
+    double2* __restrict__ const snap = S.snap;
+    double* __restrict__ const part_l = S.part_l;
     const int p = wt.p0 + lane;
     const int j = (p % P) * m + p / P;
     const bool live = j < S.n;
@@ -230,45 +235,69 @@ __global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, con
         if (do_end) sh_rotate<VEL>(f_end * c_m1, s1m, s10);
         // stage 2: E_end on (v0-2, v0-1), mask; stage 3: E_start on the same pair
         double2 s2a = carry_s1, s2b = s1m;
-        if (fuse_e) {
+        const int le = v0 - 2 - q;
+        if (both && !observe) {
+            // E_end and E_start as one rotation (the mask, a function of r only, commutes with both)
             sh_rotate<VEL>(f_sum * c_m2, s2a, s2b);
             s2a = cscale(mk, s2a);
             s2b = cscale(mk, s2b);
         } else {
-            if (do_end) {
+            // At a data time the stored rows are the same as elsewhere (one rotation when both halves
+            // apply), and the pair's norm is that of the rotated rows (E_start keeps it). Rows observed
+            // on their own -- below LB (snapshots for the inner products), the largest l, rows outside
+            // [lo, hi) (counted by the neighbouring segment), every row for the norm by l -- take E_end
+            // and the mask once more by themselves.
+            // decided for the whole wave (lanes differ only by the parity q): no divergent rotations
+            const bool per_row = __any(!both || le < LB || le < lo || le + 1 >= hi || le + 1 >= L - 1);
+            double2 oa = s2a, ob = s2b;  // the observed rows: g after step n
+            if (both) {
+                if (by_l || per_row) {
+                    sh_rotate<VEL>(f_end * c_m2, oa, ob);
+                    oa = cscale(mk, oa);
+                    ob = cscale(mk, ob);
+                }
+                sh_rotate<VEL>(f_sum * c_m2, s2a, s2b);
+                s2a = cscale(mk, s2a);
+                s2b = cscale(mk, s2b);
+            } else if (do_end) {
                 sh_rotate<VEL>(f_end * c_m2, s2a, s2b);
                 s2a = cscale(mk, s2a);
                 s2b = cscale(mk, s2b);
+                oa = s2a;
+                ob = s2b;
             }
-            const int le = v0 - 2 - q;
             if (observe && live) {
-                if (le >= lo && le < hi) {
-                    acc_norm += cabs2(s2a);
-                    if (le == L - 1) acc_lnorm += cabs2(s2a);
-                    if (le < S.LB) st2(S.snap, (size_t)le * npad + p, s2a);
-                }
-                if (le + 1 >= lo && le + 1 < hi) {
-                    acc_norm += cabs2(s2b);
-                    if (le + 1 == L - 1) acc_lnorm += cabs2(s2b);
-                    if (le + 1 < S.LB) st2(S.snap, (size_t)(le + 1) * npad + p, s2b);
+                if (!per_row) {
+                    acc_norm += cabs2(s2a) + cabs2(s2b);
+                } else {
+                    if (le >= lo && le < hi) {
+                        acc_norm += cabs2(oa);
+                        if (le == L - 1) acc_lnorm += cabs2(oa);
+                        if (le < LB) st2(snap, (size_t)le * npad + p, oa);
+                    }
+                    if (le + 1 >= lo && le + 1 < hi) {
+                        acc_norm += cabs2(ob);
+                        if (le + 1 == L - 1) acc_lnorm += cabs2(ob);
+                        if (le + 1 < LB) st2(snap, (size_t)(le + 1) * npad + p, ob);
+                    }
                 }
             }
             if (by_l) {
                 // this iteration finishes rows v0-2 (all lanes), v0-1 (q = 0 lanes)
                 // and v0-3 (q = 1 lanes); sums in a fixed order per wave
                 const bool ok_a = live && le >= lo && le < hi, ok_b = live && le + 1 >= lo && le + 1 < hi;
-                const double na = ok_a ? cabs2(s2a) : 0.0, nb = ok_b ? cabs2(s2b) : 0.0;
+                const double na = ok_a ? cabs2(oa) : 0.0, nb = ok_b ? cabs2(ob) : 0.0;
                 const double r3 = wave_sum(q ? na : 0.0);  // row v0-3
                 const double r2 = wave_sum(q ? nb : na);   // row v0-2
                 const double r1 = wave_sum(q ? 0.0 : nb);  // row v0-1
                 if (lane == 0) {
-                    double* pl = S.part_l + (size_t)wt.part * kShSegRows;
+                    double* pl = part_l + (size_t)wt.part * kShSegRows;
                     if (v0 - 3 >= lo && v0 - 3 < hi) pl[v0 - 3 - lo] = pend + r3;
                     if (v0 - 2 >= lo && v0 - 2 < hi) pl[v0 - 2 - lo] = r2;
                 }
                 pend = r1;
             }
-            if (do_start) sh_rotate<VEL>(f_start * c_m2, s2a, s2b);
+            if (!both && do_start) sh_rotate<VEL>(f_start * c_m2, s2a, s2b);
         }
         // stage 4: O_start on (v0-3, v0-2)
         double2 o_a = carry_s3, o_b = s2a;
