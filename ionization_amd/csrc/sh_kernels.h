@@ -405,18 +405,21 @@ enum { CN_CAYLEY_MEM = 0, CN_CAYLEY_LDS = 1, CN_EXPLICIT = 2 };
 // at gl (row i at gl[i * stride]; stride P by default); solves and stores
 // y = (1 + i H~)^-1 (1 - i H~) x. Ratio / ib: optional pass-2 pivot ratios
 // (cn_implicit_lowreg). LANES / ln: LANES lines per workgroup, interleaved in LDS.
+// store = false: the thread solves but writes nothing (rows that are all padding).
 template <int M, int FORM, class Coef, class Coef2, class Ratio = NoRatio, int LANES = 1>
 __device__ __forceinline__ void cn_line_finish(double2 (&x)[M], double2* __restrict__ gl, const int k, const int P,
                                                Coef coef, Coef2 coef2, double2* __restrict__ lds,
                                                Ratio ratio = Ratio{}, const int ib = -1, const int ln = 0,
-                                               const int stride = -1) {
+                                               const int stride = -1, const bool store = true) {
     const size_t st = stride < 0 ? (size_t)P : (size_t)stride;
     if constexpr (FORM == CN_EXPLICIT) {
         double2 xl, xr;
         chunk_neighbours<M, LANES>(x, k, P, lds, xl, xr, false, ln);
         cn_implicit_lowreg<M, Coef, Coef2, true, LANES, Ratio>(x, k, P, coef, coef2, lds, xl, xr, ln, ratio, ib);
+        if (store) {
 #pragma unroll
-        for (int i = 0; i < M; ++i) st2s(gl, (size_t)i * st, x[i]);
+            for (int i = 0; i < M; ++i) st2s(gl, (size_t)i * st, x[i]);
+        }
     } else {
         double2* keep = lds + 3 * P * LANES;  // CN_CAYLEY_LDS: this thread's own rows only
         if constexpr (FORM == CN_CAYLEY_LDS) {
@@ -425,6 +428,7 @@ __device__ __forceinline__ void cn_line_finish(double2 (&x)[M], double2* __restr
         }
         cn_implicit_lowreg<M, Coef, Coef2, false, LANES, Ratio>(x, k, P, coef, coef2, lds, cmk(0.0, 0.0),
                                                                 cmk(0.0, 0.0), ln, ratio, ib);
+        if (!store) return;
 #pragma unroll
         for (int i = 0; i < M; ++i) {
             const double2 x0 = (FORM == CN_CAYLEY_LDS) ? keep[(i * P + k) * LANES + ln] : ld2(gl, (size_t)i * st);
@@ -475,9 +479,18 @@ k_sh_rphase(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_line
     const int nr = S.n;
     const double a_left = (k > 0) ? ld1(S.tab, SH_AN * npad + (M - 1) * P + k - 1) : 0.0;
 
+    // A thread whose rows all lie past the last radial point (j >= n) holds padding: zero in
+    // memory, identity rows in the tables, decoupled from row n - 1 (A_{n-1} = 0). It solves
+    // with zeros and neither reads nor writes its rows.
+    const bool pad = k * M >= nr;
     double2 x[M];
+    if (pad) {
 #pragma unroll
-    for (int i = 0; i < M; ++i) x[i] = (FORM == CN_CAYLEY_MEM) ? ld2(gl, (size_t)i * P) : ld2s(gl, (size_t)i * P);
+        for (int i = 0; i < M; ++i) x[i] = cmk(0.0, 0.0);
+    } else {
+#pragma unroll
+        for (int i = 0; i < M; ++i) x[i] = (FORM == CN_CAYLEY_MEM) ? ld2(gl, (size_t)i * P) : ld2s(gl, (size_t)i * P);
+    }
     auto diag = [&](int i, double T) {
         const int j = k * M + i;
         const double r = fma((double)j, dr, 0.5 * dr);
@@ -504,7 +517,7 @@ k_sh_rphase(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_line
         hm = sn * carry2;
         carry2 = tv.y;
     };
-    cn_line_finish<M, FORM>(x, gl, k, P, coef, coef2, lds);
+    cn_line_finish<M, FORM>(x, gl, k, P, coef, coef2, lds, NoRatio{}, -1, 0, -1, !pad);
 }
 
 // ------------------------------------------------------------------ velocity gauge: h2
