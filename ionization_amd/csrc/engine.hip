@@ -472,7 +472,7 @@ void dispatch_adi_r(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
         case 10: b.P > 256 ? launch_adi_r<10, 512, 1, EXPL>(ctx, b, st) : launch_adi_r<10, 256, 3, EXPL>(ctx, b, st); break;
         case 12: b.P > 256 ? launch_adi_r<12, 512, 1, EXPL>(ctx, b, st) : launch_adi_r<12, 256, 2, EXPL>(ctx, b, st); break;
         case 14: b.P > 256 ? launch_adi_r<14, 512, 1, EXPL>(ctx, b, st) : launch_adi_r<14, 256, 2, EXPL>(ctx, b, st); break;
-        default: b.P > 256 ? launch_adi_r<16, 512, 1, EXPL>(ctx, b, st) : launch_adi_r<16, 256, 1, EXPL>(ctx, b, st); break;
+        default: b.P > 256 ? launch_adi_r<16, 512, 1, EXPL>(ctx, b, st) : launch_adi_r<16, 256, 2, EXPL>(ctx, b, st); break;
     }
 }
 

@@ -610,15 +610,24 @@ __global__ void __launch_bounds__(512) k_sh_vphase(ShSim* __restrict__ sims, con
     double2* __restrict__ g1 = g0 + npad;
     const double* __restrict__ av = S.tab + (size_t)SH_AV * npad;
     constexpr double kRsqrt2 = 0.70710678118654752440;
+    // all rows padding (j >= n): zeros with zero angles (alpha_j = 0 from j = n - 1 on), neither
+    // read nor written
+    const bool pad = k * M >= S.n;
     double2 s[M], d[M];
+    if (pad) {
 #pragma unroll
-    for (int i = 0; i < M; ++i) {
-        const double2 a = ld2s(g0, (size_t)i * P), b = ld2s(g1, (size_t)i * P);
-        s[i] = cmk((a.x + b.x) * kRsqrt2, (a.y + b.y) * kRsqrt2);
-        d[i] = cmk((a.x - b.x) * kRsqrt2, (a.y - b.y) * kRsqrt2);
+        for (int i = 0; i < M; ++i) s[i] = d[i] = cmk(0.0, 0.0);
+    } else {
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            const double2 a = ld2s(g0, (size_t)i * P), b = ld2s(g1, (size_t)i * P);
+            s[i] = cmk((a.x + b.x) * kRsqrt2, (a.y + b.y) * kRsqrt2);
+            d[i] = cmk((a.x - b.x) * kRsqrt2, (a.y - b.y) * kRsqrt2);
+        }
     }
     sh_vel_rpairs<M>(s, d, av, f, P, k, REV, ex);
     sh_vel_rpairs<M>(s, d, av, f, P, k, !REV, ex);
+    if (pad) return;
 #pragma unroll
     for (int i = 0; i < M; ++i) {
         st2s(g0, (size_t)i * P, cmk((s[i].x + d[i].x) * kRsqrt2, (s[i].y + d[i].y) * kRsqrt2));
@@ -738,9 +747,15 @@ k_sh_rphase_vpair(ShSim* __restrict__ sims, const int2* __restrict__ pairs, int 
     const double fp = fh * ld1(cl, pt.y);  // the pair's angle factor (row l's combination +theta, row l + 1's -theta)
     const double* __restrict__ av = S.tab + (size_t)SH_AV * npad;
 
+    const bool pad = k * M >= nr;  // all rows padding: zeros, neither read nor written (as in the R-phase)
     double2 x[M];
+    if (pad) {
 #pragma unroll
-    for (int i = 0; i < M; ++i) x[i] = ld2s(gl, (size_t)i * P);
+        for (int i = 0; i < M; ++i) x[i] = cmk(0.0, 0.0);
+    } else {
+#pragma unroll
+        for (int i = 0; i < M; ++i) x[i] = ld2s(gl, (size_t)i * P);
+    }
     if (fh != 0.0) sh_vpair_factors<M>(x, k, P, ln, lds, av, fp, false);
 
     auto diag = [&](int i, double T) {
@@ -773,6 +788,7 @@ k_sh_rphase_vpair(ShSim* __restrict__ sims, const int2* __restrict__ pairs, int 
     chunk_neighbours<M, 2>(x, k, P, lds, xl, xr, true, ln);
     cn_implicit_lowreg<M, decltype(coef), decltype(coef2), true, 2, NoRatio, true>(x, k, P, coef, coef2, lds, xl, xr, ln);
     if (fh != 0.0) sh_vpair_factors<M>(x, k, P, ln, lds, av, fp, true);
+    if (pad) return;
 #pragma unroll
     for (int i = 0; i < M; ++i) st2s(gl, (size_t)i * P, x[i]);
 }
@@ -813,9 +829,15 @@ k_sh_adi_r(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines
     const int nr = S.n;
     const double a_left = (k > 0) ? ld1(S.tab, SH_AN * npad + (M - 1) * P + k - 1) : 0.0;
 
+    const bool pad = k * M >= nr;  // all rows padding: zeros, neither read nor written (as in the R-phase)
     double2 x[M];
+    if (pad) {
 #pragma unroll
-    for (int i = 0; i < M; ++i) x[i] = ld2s(gl, (size_t)i * P);
+        for (int i = 0; i < M; ++i) x[i] = cmk(0.0, 0.0);
+    } else {
+#pragma unroll
+        for (int i = 0; i < M; ++i) x[i] = ld2s(gl, (size_t)i * P);
+    }
     auto diag = [&](int i, double T) {
         const int j = k * M + i;
         const double r = fma((double)j, dr, 0.5 * dr);
@@ -855,6 +877,7 @@ k_sh_adi_r(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines
         };
         cn_implicit_lowreg<M, decltype(coef), decltype(coef2), false, 1>(x, k, P, coef, coef2, lds);
     }
+    if (pad) return;
 #pragma unroll
     for (int i = 0; i < M; ++i) st2s(gl, (size_t)i * P, x[i]);
 }
