@@ -185,6 +185,7 @@ struct tdse_ctx {
     std::vector<Bucket> cyl_r_buckets;
     double* d_cl = nullptr;
     double* d_clv = nullptr;  // c_l (l + 1): the velocity gauge's h1 coupling per l
+    int* d_ctl = nullptr;     // [1] nonzero while the next replay of the current run follows (ADI run-ahead)
     DevVec<ShWave> waves;     // length-gauge simulations' waves first, then velocity-gauge ones
     int n_len_waves = 0;
     DevVec<int2> jobs;
@@ -458,7 +459,7 @@ template <int M, int MAXT, int WPE, bool EXPL>
 void launch_adi_r(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
     const size_t lds = 3 * (size_t)b.P * sizeof(double2);
     hipLaunchKernelGGL((k_sh_adi_r<M, MAXT, WPE, EXPL>), dim3((unsigned)b.lines.count), dim3(b.P), lds, st,
-                       ctx->d_sh, b.lines.ptr, (int)b.lines.count);
+                       ctx->d_sh, b.lines.ptr, (int)b.lines.count, ctx->d_ctl);
 }
 
 // the R-phase's launch shapes
@@ -468,7 +469,7 @@ void dispatch_adi_r(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
         case 2: launch_adi_r<2, 512, 4, EXPL>(ctx, b, st); break;
         case 4: launch_adi_r<4, 512, 4, EXPL>(ctx, b, st); break;
         case 6: launch_adi_r<6, 512, 4, EXPL>(ctx, b, st); break;
-        case 8: launch_adi_r<8, 512, 4, EXPL>(ctx, b, st); break;
+        case 8: launch_adi_r<8, 512, EXPL ? 4 : 3, EXPL>(ctx, b, st); break;  // the run-ahead's registers
         case 10: b.P > 256 ? launch_adi_r<10, 512, 1, EXPL>(ctx, b, st) : launch_adi_r<10, 256, 3, EXPL>(ctx, b, st); break;
         case 12: b.P > 256 ? launch_adi_r<12, 512, 1, EXPL>(ctx, b, st) : launch_adi_r<12, 256, 2, EXPL>(ctx, b, st); break;
         case 14: b.P > 256 ? launch_adi_r<14, 512, 1, EXPL>(ctx, b, st) : launch_adi_r<14, 256, 2, EXPL>(ctx, b, st); break;
@@ -713,7 +714,7 @@ int launch_step(tdse_ctx* ctx, bool cyl_pre) {
                 return rc;
         }
         const int ns = (int)ctx->sh.size();
-        hipLaunchKernelGGL(k_sh_advance, dim3((ns + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh, ns);
+        hipLaunchKernelGGL(k_sh_advance, dim3((ns + 255) / 256), dim3(256), 0, ctx->stream, ctx->d_sh, ns, ctx->d_ctl);
     }
     for (auto& b : ctx->line_buckets) dispatch_line(ctx, b, 1);
     if (!ctx->cyl_r_buckets.empty()) {
@@ -866,6 +867,7 @@ int tdse_ctx_create(int device, size_t pool_bytes, tdse_ctx** out) {
               hipMalloc(&ctx->d_clv, kMaxL * sizeof(double)) == hipSuccess &&
               hipMemcpy(ctx->d_clv, clv.data(), kMaxL * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
               hipMalloc(&ctx->d_sh, kSlotCapacity * sizeof(ShSim)) == hipSuccess &&
+              hipMalloc(&ctx->d_ctl, sizeof(int)) == hipSuccess && hipMemset(ctx->d_ctl, 0, sizeof(int)) == hipSuccess &&
               hipMalloc(&ctx->d_line, kSlotCapacity * sizeof(LineSim)) == hipSuccess &&
               hipMalloc(&ctx->d_cyl, kSlotCapacity * sizeof(CylSim)) == hipSuccess &&
               hipMemset(ctx->d_sh, 0, kSlotCapacity * sizeof(ShSim)) == hipSuccess &&
@@ -906,6 +908,7 @@ int tdse_ctx_destroy(tdse_ctx* ctx) {
     if (ctx->d_cl) (void)hipFree(ctx->d_cl);
     if (ctx->d_clv) (void)hipFree(ctx->d_clv);
     if (ctx->d_sh) (void)hipFree(ctx->d_sh);
+    if (ctx->d_ctl) (void)hipFree(ctx->d_ctl);
     if (ctx->d_line) (void)hipFree(ctx->d_line);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
@@ -1381,16 +1384,25 @@ int tdse_run(tdse_ctx* ctx, int32_t n_steps, int32_t use_graph) {
         if (h.live && h.dev.hold) hold_any = true;
     ctx->ev_used = 0;
     ctx->ev_pairs.clear();
+    // SH ADI simulations may run ahead into the next step on every replay of this run but
+    // the last, so that the run ends with every simulation at an ordinary step boundary
+    // (mask pending) which the host can materialise, download or checkpoint.
+    const bool ahead = !ctx->sh_adi_buckets.empty() && n_steps > 1;
     HIP_TRY(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
+    if (!ctx->sh_adi_buckets.empty())
+        HIP_TRY(ctx, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(ctx->d_ctl), ahead ? 1 : 0, 1, ctx->stream));
     for (int i = 0; i < n_steps;) {
         // the pre-sweep is needed for simulations held at exact states, and for
         // one step after a cylindrical simulation was added or uploaded
         int done = 1;
+        const int left = n_steps - i - (ahead ? 1 : 0);  // steps before the last one of a run with run-ahead
+        if (ahead && left == 0)
+            HIP_TRY(ctx, hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(ctx->d_ctl), 0, 1, ctx->stream));
         if (ctx->cyl_pending && !hold_any) {
             if ((rc = launch_step(ctx, true))) return rc;
         } else if (graph) {
             if ((rc = ensure_graph(ctx, hold_any))) return rc;
-            if (kGraphSteps > 1 && n_steps - i >= kGraphSteps) {
+            if (kGraphSteps > 1 && left >= kGraphSteps) {
                 HIP_TRY(ctx, hipGraphLaunch(ctx->exec_k, ctx->stream));
                 done = kGraphSteps;
             } else {

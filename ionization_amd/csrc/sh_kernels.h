@@ -54,6 +54,8 @@ struct ShSim {
     int adi;               // 1: AlternatingDirectionImplicit (fs is then zero: the L-phase only masks and observes)
     const double* fa;      // ADI: [N] field times tau ratio per step, E(t_{n+1} + dt/2) tau_n / tau0
     int flags;             // health flags (norm_health), device-updated at data times
+    int ahead;             // ADI: 1 after a replay whose implicit r-sweep also applied the mask of its step
+                           // and the explicit H0 half of the next one (device-updated; 0 between runs)
 };
 
 constexpr int kShSegRows = 64;  // most rows of one L-phase wave (its l-segment)
@@ -77,6 +79,13 @@ __device__ __forceinline__ bool sh_is_data_step(int n, int N, int every) {
 __device__ __forceinline__ int sh_data_slot(int n, int N, int every) {
     if (every >= 1) return (n == N && (N % every) != 0) ? N / every + 1 : n / every;
     return n == 0 ? 0 : 1;
+}
+// AlternatingDirectionImplicit: may the implicit r-sweep of step n run ahead into step n + 1
+// (apply the mask that ends step n and the explicit H0 half of step n + 1)? Only when the
+// next replay of the same run follows (ok), step n + 1 starts there (n + 1 < N) and the g
+// after n + 1 steps is not observed.
+__device__ __forceinline__ bool sh_adi_ahead(const ShSim& S, int n, int ok) {
+    return ok && n + 1 < S.N && !sh_is_data_step(n + 1, S.N, S.store_every);
 }
 
 // Length gauge (VEL = false): exp(-i a sigma_x) applied to (x0, x1),
@@ -150,6 +159,7 @@ __global__ void __launch_bounds__(256) k_sh_lphase(ShSim* __restrict__ sims, con
     const int n = S.step;
     const int N = S.N;
     if (!S.active || n > N) return;
+    if (S.ahead) return;  // ADI: the previous replay already masked g and started this step in place of it
     const bool do_end = n > 0 && S.phase == 0;
     const bool do_start = n < N && !end_only;
     const bool observe = !end_only && sh_is_data_step(n, N, S.store_every);
@@ -803,13 +813,20 @@ k_sh_rphase_vpair(ShSim* __restrict__ sims, const int2* __restrict__ pairs, int 
 // the L-phase (with zero rotation angles) applies the mask and observes at data
 // times, then three in-place sweeps of sh_state(n + 1) follow: k_sh_adi_r
 // (explicit H0), k_sh_adi_l (the l-direction Cayley factor), k_sh_adi_r
-// (implicit H0).
+// (implicit H0). Within a run, where the next step's g is not observed, the
+// implicit sweep also applies the mask and the next explicit half, and the next
+// replay runs only the l-direction factor and the implicit sweep: two sweeps per
+// step instead of four.
 
 // H0 along r for one l-line (the R-phase's layout and coefficients):
-// EXPL: y = (1 - i H~) x; otherwise y = (1 + i H~)^-1 x.
+// EXPL: y = (1 - i H~) x; otherwise y = (1 + i H~)^-1 x. Run ahead (implicit sweep
+// only, sh_adi_ahead): then also y <- (1 - i H~_{n+1}) M y, the mask that ends step n
+// and the explicit half of step n + 1, written to the state buffer of step n + 2; the
+// next replay skips its L-phase and explicit sweep for the simulation (S.ahead). The
+// products are the ones those two sweeps would form.
 template <int M, int MAXT, int WPE, bool EXPL>
 __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
-k_sh_adi_r(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines) {
+k_sh_adi_r(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines, const int* __restrict__ ctl) {
     extern __shared__ double2 lds[];  // 3 * P
     const int line = xcd_remap(blockIdx.x, gridDim.x);
     if (line >= n_lines) return;
@@ -817,6 +834,7 @@ k_sh_adi_r(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines
     ShSim& S = sims[lt.x];
     const int n = S.step;
     if (!S.active || n >= S.N) return;  // uniform across the block
+    if (EXPL && S.ahead) return;         // applied by the previous replay's implicit sweep
     const int l = lt.y;
     const int P = S.P, npad = S.npad;
     const int k = threadIdx.x;
@@ -838,44 +856,70 @@ k_sh_adi_r(ShSim* __restrict__ sims, const int2* __restrict__ lines, int n_lines
 #pragma unroll
         for (int i = 0; i < M; ++i) x[i] = ld2s(gl, (size_t)i * P);
     }
-    auto diag = [&](int i, double T) {
-        const int j = k * M + i;
-        const double r = fma((double)j, dr, 0.5 * dr);
-        const double U = (j < nr) ? cu * tdse_rcp(r * r) : 0.0;
-        return sn * (fma(ll1, U, T) + (i == 0 ? c0 : 0.0));
+    // s (T_j + l(l+1) U_j [+ corr0]): the diagonal of s H in tau0 units (thread kk, grid step rd, factor cc)
+    auto diag = [&](int i, double T, double s, int kk, double rd, double cc) {
+        const int j = kk * M + i;
+        const double r = fma((double)j, rd, 0.5 * rd);
+        const double U = (j < nr) ? cc * tdse_rcp(r * r) : 0.0;
+        return s * (fma(ll1, U, T) + (i == 0 ? c0 : 0.0));
     };
-    double carry = a_left;
-    auto coef = [&](int i, double& hm, double& h, double& hp) {
-        const double2 tv = ld2(TA, (size_t)i * P);
-        h = diag(i, tv.x);
-        hm = sn * carry;
-        hp = sn * tv.y;
-        carry = tv.y;
-    };
-    if constexpr (EXPL) {
+    // x <- (1 - i s H) x with the chunk neighbours' edge rows. Opaque copies of the table
+    // pointer and the grid constants keep the compiler from holding the solver's
+    // first-pass coefficients (the same loads and values) live across the solve for a
+    // run-ahead; the values are the same.
+    auto explicit_h0 = [&](double s, bool sync_first) {
         double2 xl, xr;
-        chunk_neighbours<M>(x, k, P, lds, xl, xr, false);
+        chunk_neighbours<M>(x, k, P, lds, xl, xr, sync_first);
         double2 prev = xl;
+        const double2* TA3 = TA;
+        double dr3 = dr, cu3 = cu;
+        int k3 = k;
+        asm volatile("" : "+v"(TA3), "+v"(dr3), "+v"(cu3), "+v"(k3));
+        double carry = (k3 > 0) ? ld1(S.tab, SH_AN * npad + (M - 1) * P + k3 - 1) : 0.0;  // a_left, reloaded
 #pragma unroll
         for (int i = 0; i < M; ++i) {
-            double hm, h, hp;
-            coef(i, hm, h, hp);
+            const double2 tv = ld2(TA3, (size_t)i * P);
+            const double h = diag(i, tv.x, s, k3, dr3, cu3), hm = s * carry, hp = s * tv.y;
+            carry = tv.y;
             const double2 xp = (i + 1 < M) ? x[i + 1 < M ? i + 1 : i] : xr;
             const double2 hx = cmk(fma(hm, prev.x, fma(h, x[i].x, hp * xp.x)), fma(hm, prev.y, fma(h, x[i].y, hp * xp.y)));
             prev = x[i];
             x[i] = cmk(x[i].x + hx.y, x[i].y - hx.x);  // x - i H~ x
         }
+    };
+    if constexpr (EXPL) {
+        explicit_h0(sn, false);
     } else {
+        double carry = a_left;
+        auto coef = [&](int i, double& hm, double& h, double& hp) {
+            const double2 tv = ld2(TA, (size_t)i * P);
+            h = diag(i, tv.x, sn, k, dr, cu);
+            hm = sn * carry;
+            hp = sn * tv.y;
+            carry = tv.y;
+        };
         double carry2 = a_left;
         auto coef2 = [&](int i, double& hm, double& h) {
             const double2* TA2 = TA;
             asm volatile("" : "+v"(TA2));
             const double2 tv = ld2(TA2, (size_t)i * P);
-            h = diag(i, tv.x);
+            h = diag(i, tv.x, sn, k, dr, cu);
             hm = sn * carry2;
             carry2 = tv.y;
         };
         cn_implicit_lowreg<M, decltype(coef), decltype(coef2), false, 1>(x, k, P, coef, coef2, lds);
+        if (sh_adi_ahead(S, n, *ctl)) {  // uniform across the block
+            const double* mk = S.tab + (size_t)SH_MASK * npad + k;
+            asm volatile("" : "+v"(mk));  // loaded after the solve, not hoisted into it
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                x[i] = cscale(ld1(mk, (size_t)i * P), x[i]);
+                // the rounded product, as the L-phase stores it: not contracted into the next sums
+                asm volatile("" : "+v"(x[i].x), "+v"(x[i].y));
+            }
+            explicit_h0(ld1(S.ts, n + 1), true);  // the solver's LDS rows may still be read
+            gl = sh_state(S, n + 2) + (size_t)l * npad + k;
+        }
     }
     if (pad) return;
 #pragma unroll
@@ -993,10 +1037,13 @@ __global__ void __launch_bounds__(kAdiLanes * kAdiChunks) k_sh_adi_lp(ShSim* __r
 }
 
 // Advance every active simulation's step counter after a full graph step.
-__global__ void k_sh_advance(ShSim* sims, int n_sims) {
+__global__ void k_sh_advance(ShSim* sims, int n_sims, const int* __restrict__ ctl) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < n_sims && sims[s].active && sims[s].step <= sims[s].N) {
-        sims[s].step += 1;
-        sims[s].phase = 0;
+        ShSim& S = sims[s];
+        // the same decision as this replay's implicit ADI r-sweep
+        S.ahead = (S.adi && S.step < S.N && sh_adi_ahead(S, S.step, *ctl)) ? 1 : 0;
+        S.step += 1;
+        S.phase = 0;
     }
 }

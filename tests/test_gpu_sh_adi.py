@@ -74,6 +74,49 @@ def test_sh_adi_norm_by_l_sparse_data_and_direct_launches(device_ctx):
     assert np.array_equal(direct.g, graph.g)
 
 
+@pytest.mark.parametrize("store_every", [1, 9, 1000])
+def test_sh_adi_run_ahead_matches_single_steps(device_ctx, store_every):
+    """On every replay of a run but the last, the implicit r-sweep may run ahead into the next
+    step (the mask, the next explicit half) and the next replay skips its L-phase and explicit
+    sweep. One run, runs in uneven chunks with a materialised state read in between, and one
+    replay per run (never ahead) give the same bits; every store cadence against the oracle."""
+    from ionization_amd import engine
+
+    case = sh_case(r_points=201, l_bound=21, t_half_as=50.0, store_every=store_every, method="adi")
+    comp = case["compiled"]
+    total = comp.n_steps + 1
+    one = _run(device_ctx, comp)
+    _check(one, case)
+
+    def chunked(chunks, mid):
+        slot = device_ctx.add_sh(comp)
+        done, i, g_mid = 0, 0, None
+        while done < total:
+            k = min(chunks[i % len(chunks)], total - done)
+            if done < mid < done + k:
+                k = mid - done
+            device_ctx.run(k)
+            done += k
+            i += 1
+            if done == mid:
+                device_ctx.materialize(engine.KIND_SH, slot)
+                assert device_ctx.state(engine.KIND_SH, slot)[0] == mid
+                g_mid = device_ctx.download_sh(slot).g
+        assert device_ctx.finished(engine.KIND_SH, slot)
+        res = device_ctx.download_sh(slot)
+        device_ctx.remove(engine.KIND_SH, slot)
+        return res, g_mid
+
+    mid = total // 2 + 1
+    single, g_single = chunked([1], mid)
+    uneven, g_uneven = chunked([3, 8, 2, 17], mid)
+    print(f"SH ADI run-ahead, store_every={store_every}: {comp.n_steps} steps, state read at {mid}", flush=True)
+    for r in (single, uneven):
+        assert np.array_equal(r.g, one.g) and np.array_equal(r.norms, one.norms)
+        assert np.array_equal(r.lnorms, one.lnorms) and np.array_equal(r.ips, one.ips)
+    assert np.array_equal(g_uneven, g_single)
+
+
 def test_sh_methods_in_one_batch(device_ctx):
     """ADI, split operator and velocity gauge in one context: each bit-identical to its run alone."""
     cases = [sh_case(r_points=200, l_bound=20, t_half_as=50.0, method="adi"),
