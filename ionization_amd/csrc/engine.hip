@@ -633,8 +633,12 @@ int launch_step(tdse_ctx* ctx, bool cyl_pre) {
                                ctx->waves.ptr + nl, nv, ctx->d_clv, 0);
         }
         prof_end(ctx);
+        // The observables read only what the L-phase wrote (per-wave partials, snapshot rows), which
+        // nothing else touches before the next replay: they can run beside the R-phase.
         const int nj = (int)ctx->jobs.count;
-        hipLaunchKernelGGL(k_sh_observe, dim3(nj), dim3(256), 0, ctx->stream, ctx->d_sh, ctx->jobs.ptr, nj);
+        auto observe = [&](hipStream_t st) {
+            hipLaunchKernelGGL(k_sh_observe, dim3(nj), dim3(256), 0, st, ctx->d_sh, ctx->jobs.ptr, nj);
+        };
         // One bucket's chain: velocity-gauge h2 factors of its even, then odd l-pairs
         // (before the CN solve), the R-phase, the factors in reverse (after it). The
         // chains of different buckets touch different simulations.
@@ -663,6 +667,7 @@ int launch_step(tdse_ctx* ctx, bool cyl_pre) {
                 }
         };
         if (ctx->profiling || ctx->sh_buckets.size() < 2) {
+            observe(ctx->stream);
             for (auto& b : ctx->sh_buckets) chain(b, ctx->stream);
         } else {
             // independent buckets run concurrently (largest first, round robin
@@ -670,6 +675,8 @@ int launch_step(tdse_ctx* ctx, bool cyl_pre) {
             // buckets fill the tails of large ones; fork and join by events
             HIP_TRY(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
             for (int j = 0; j < kSideStreams; ++j) HIP_TRY(ctx, hipStreamWaitEvent(ctx->side[j], ctx->ev_fork, 0));
+            // the observables on the last side stream, which the round robin leaves the least work
+            observe(ctx->side[kSideStreams - 1]);
             for (size_t i = 0; i < ctx->sh_order.size(); ++i) {
                 const int j = (int)(i % (kSideStreams + 1));
                 chain(ctx->sh_buckets[ctx->sh_order[i]], j == 0 ? ctx->stream : ctx->side[j - 1]);
