@@ -519,22 +519,25 @@ void dispatch_rphase_vpair(tdse_ctx* ctx, const Bucket& b, hipStream_t st) {
     }
 }
 
+// Two threads per chunk (each with half its rows) where the halves keep an even row count and
+// the workgroup stays within 512 threads: half the registers, twice the waves.
 template <bool REV>
 void dispatch_vphase(tdse_ctx* ctx, const Bucket& b, int par, hipStream_t st) {
     const DevVec<int2>& pairs = b.vpairs[par];
-    const dim3 grid((unsigned)pairs.count), block(b.P);
+    const bool split = (b.M == 8 || b.M == 12 || b.M == 16) && 2 * b.P <= 512;
+    const dim3 grid((unsigned)pairs.count), block((unsigned)(split ? 2 * b.P : b.P));
     auto args = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, grid, block, 0, st, ctx->d_sh, pairs.ptr, (int)pairs.count, ctx->d_cl);
     };
     switch (b.M) {
-        case 2: args(k_sh_vphase<2, REV>); break;
-        case 4: args(k_sh_vphase<4, REV>); break;
-        case 6: args(k_sh_vphase<6, REV>); break;
-        case 8: args(k_sh_vphase<8, REV>); break;
-        case 10: args(k_sh_vphase<10, REV>); break;
-        case 12: args(k_sh_vphase<12, REV>); break;
-        case 14: args(k_sh_vphase<14, REV>); break;
-        default: args(k_sh_vphase<16, REV>); break;
+        case 2: args(k_sh_vphase<2, REV, 1>); break;
+        case 4: args(k_sh_vphase<4, REV, 1>); break;
+        case 6: args(k_sh_vphase<6, REV, 1>); break;
+        case 8: split ? args(k_sh_vphase<8, REV, 2>) : args(k_sh_vphase<8, REV, 1>); break;
+        case 10: args(k_sh_vphase<10, REV, 1>); break;
+        case 12: split ? args(k_sh_vphase<12, REV, 2>) : args(k_sh_vphase<12, REV, 1>); break;
+        case 14: args(k_sh_vphase<14, REV, 1>); break;
+        default: split ? args(k_sh_vphase<16, REV, 2>) : args(k_sh_vphase<16, REV, 1>); break;
     }
 }
 

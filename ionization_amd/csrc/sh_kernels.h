@@ -570,39 +570,57 @@ __device__ __forceinline__ void sh_vel_pair(double th, double2& s0, double2& s1,
     asm("" : "+v"(s0.x), "+v"(s0.y), "+v"(s1.x), "+v"(s1.y), "+v"(d0.x), "+v"(d0.y), "+v"(d1.x), "+v"(d1.y));
 }
 
-template <int M>
-__device__ __forceinline__ void sh_vel_rpairs(double2 (&s)[M], double2 (&d)[M], const double* __restrict__ av,
-                                              const double f, const int P, const int k, const bool odd,
-                                              double2 (*ex)[512]) {
+// Position of row i of "virtual chunk" c when SPLIT threads share each chunk of M rows:
+// thread c holds MH = M / SPLIT consecutive radial rows j = c MH + i, rows h MH + i of
+// chunk k = c / SPLIT (h = c % SPLIT), at position (h MH + i) P + k.
+template <int M, int SPLIT>
+__device__ __forceinline__ size_t sh_vel_pos(const int c, const int i, const int P) {
+    constexpr int MH = M / SPLIT;
+    return (size_t)((c % SPLIT) * MH + i) * P + c / SPLIT;
+}
+
+// The radial pairs of one parity for thread c's MH rows (MH even: the even-j pairs lie inside
+// the thread; of the odd-j pairs the ones across the thread's edges, (c MH - 1, c MH) and
+// (c MH + MH - 1, (c + 1) MH), go through LDS). nc = SPLIT P threads.
+template <int M, int SPLIT>
+__device__ __forceinline__ void sh_vel_rpairs(double2 (&s)[M / SPLIT], double2 (&d)[M / SPLIT],
+                                              const double* __restrict__ av, const double f, const int P,
+                                              const int c, const bool odd, double2 (*ex)[512]) {
+    constexpr int MH = M / SPLIT;
+    static_assert(MH % 2 == 0, "the even-j pairs must lie inside a thread");
+    const int nc = SPLIT * P;
+    auto pos = [&](int cc, int i) { return sh_vel_pos<M, SPLIT>(cc, i, P); };
     if (!odd) {
 #pragma unroll
-        for (int i = 0; i < M; i += 2) sh_vel_pair(f * ld1(av, (size_t)i * P + k), s[i], s[i + 1], d[i], d[i + 1]);
+        for (int i = 0; i < MH; i += 2) sh_vel_pair(f * ld1(av, pos(c, i)), s[i], s[i + 1], d[i], d[i + 1]);
         return;
     }
 #pragma unroll
-    for (int i = 1; i + 1 < M; i += 2) sh_vel_pair(f * ld1(av, (size_t)i * P + k), s[i], s[i + 1], d[i], d[i + 1]);
-    ex[0][k] = s[M - 1];
-    ex[1][k] = d[M - 1];
-    ex[2][k] = s[0];
-    ex[3][k] = d[0];
+    for (int i = 1; i + 1 < MH; i += 2) sh_vel_pair(f * ld1(av, pos(c, i)), s[i], s[i + 1], d[i], d[i + 1]);
+    ex[0][c] = s[MH - 1];
+    ex[1][c] = d[MH - 1];
+    ex[2][c] = s[0];
+    ex[3][c] = d[0];
     __syncthreads();
-    // the edge pair to the right: (my row M - 1, the next thread's row 0)
-    double2 sr = (k + 1 < P) ? ex[2][k + 1] : cmk(0.0, 0.0), dr = (k + 1 < P) ? ex[3][k + 1] : cmk(0.0, 0.0);
-    const double thr = (k + 1 < P) ? f * ld1(av, (size_t)(M - 1) * P + k) : 0.0;
-    sh_vel_pair(thr, s[M - 1], sr, d[M - 1], dr);
-    // the edge pair to the left: (the previous thread's row M - 1, my row 0)
-    double2 sl = (k > 0) ? ex[0][k - 1] : cmk(0.0, 0.0), dl = (k > 0) ? ex[1][k - 1] : cmk(0.0, 0.0);
-    const double thl = (k > 0) ? f * ld1(av, (size_t)(M - 1) * P + k - 1) : 0.0;
+    // the edge pair to the right: (my row MH - 1, the next thread's row 0)
+    double2 sr = (c + 1 < nc) ? ex[2][c + 1] : cmk(0.0, 0.0), dr = (c + 1 < nc) ? ex[3][c + 1] : cmk(0.0, 0.0);
+    const double thr = (c + 1 < nc) ? f * ld1(av, pos(c, MH - 1)) : 0.0;
+    sh_vel_pair(thr, s[MH - 1], sr, d[MH - 1], dr);
+    // the edge pair to the left: (the previous thread's row MH - 1, my row 0)
+    double2 sl = (c > 0) ? ex[0][c - 1] : cmk(0.0, 0.0), dl = (c > 0) ? ex[1][c - 1] : cmk(0.0, 0.0);
+    const double thl = (c > 0) ? f * ld1(av, pos(c > 0 ? c - 1 : 0, MH - 1)) : 0.0;
     sh_vel_pair(thl, sl, s[0], dl, d[0]);
 }
 
 // REV = false: the half before the CN solve (even-j pairs, then odd-j);
 // REV = true: the half after it (odd-j, then even-j). The launch covers the
 // l-pairs of one parity; the host orders the launches (even l-pairs, odd
-// l-pairs) before the solve and (odd, even) after it.
-template <int M, bool REV>
+// l-pairs) before the solve and (odd, even) after it. SPLIT threads per chunk
+// (SPLIT P per workgroup), each with M / SPLIT rows: fewer registers, more waves.
+template <int M, bool REV, int SPLIT>
 __global__ void __launch_bounds__(512) k_sh_vphase(ShSim* __restrict__ sims, const int2* __restrict__ pairs,
                                                    int n_pairs, const double* __restrict__ cl) {
+    constexpr int MH = M / SPLIT;
     __shared__ double2 ex[4][512];
     const int pair = xcd_remap(blockIdx.x, gridDim.x);
     if (pair >= n_pairs) return;
@@ -614,34 +632,36 @@ __global__ void __launch_bounds__(512) k_sh_vphase(ShSim* __restrict__ sims, con
     if (fn == 0.0) return;  // every angle is zero: the factor is the identity
     const int l = pt.y;
     const int P = S.P, npad = S.npad;
-    const int k = threadIdx.x;
+    const int c = threadIdx.x;
     const double f = fn * ld1(cl, l);
-    double2* __restrict__ g0 = sh_state(S, n + 1) + (size_t)l * npad + k;
+    double2* __restrict__ g0 = sh_state(S, n + 1) + (size_t)l * npad;
     double2* __restrict__ g1 = g0 + npad;
     const double* __restrict__ av = S.tab + (size_t)SH_AV * npad;
     constexpr double kRsqrt2 = 0.70710678118654752440;
     // all rows padding (j >= n): zeros with zero angles (alpha_j = 0 from j = n - 1 on), neither
     // read nor written
-    const bool pad = k * M >= S.n;
-    double2 s[M], d[M];
+    const bool pad = c * MH >= S.n;
+    double2 s[MH], d[MH];
     if (pad) {
 #pragma unroll
-        for (int i = 0; i < M; ++i) s[i] = d[i] = cmk(0.0, 0.0);
+        for (int i = 0; i < MH; ++i) s[i] = d[i] = cmk(0.0, 0.0);
     } else {
 #pragma unroll
-        for (int i = 0; i < M; ++i) {
-            const double2 a = ld2s(g0, (size_t)i * P), b = ld2s(g1, (size_t)i * P);
+        for (int i = 0; i < MH; ++i) {
+            const size_t q = sh_vel_pos<M, SPLIT>(c, i, P);
+            const double2 a = ld2s(g0, q), b = ld2s(g1, q);
             s[i] = cmk((a.x + b.x) * kRsqrt2, (a.y + b.y) * kRsqrt2);
             d[i] = cmk((a.x - b.x) * kRsqrt2, (a.y - b.y) * kRsqrt2);
         }
     }
-    sh_vel_rpairs<M>(s, d, av, f, P, k, REV, ex);
-    sh_vel_rpairs<M>(s, d, av, f, P, k, !REV, ex);
+    sh_vel_rpairs<M, SPLIT>(s, d, av, f, P, c, REV, ex);
+    sh_vel_rpairs<M, SPLIT>(s, d, av, f, P, c, !REV, ex);
     if (pad) return;
 #pragma unroll
-    for (int i = 0; i < M; ++i) {
-        st2s(g0, (size_t)i * P, cmk((s[i].x + d[i].x) * kRsqrt2, (s[i].y + d[i].y) * kRsqrt2));
-        st2s(g1, (size_t)i * P, cmk((s[i].x - d[i].x) * kRsqrt2, (s[i].y - d[i].y) * kRsqrt2));
+    for (int i = 0; i < MH; ++i) {
+        const size_t q = sh_vel_pos<M, SPLIT>(c, i, P);
+        st2s(g0, q, cmk((s[i].x + d[i].x) * kRsqrt2, (s[i].y + d[i].y) * kRsqrt2));
+        st2s(g1, q, cmk((s[i].x - d[i].x) * kRsqrt2, (s[i].y - d[i].y) * kRsqrt2));
     }
 }
 
