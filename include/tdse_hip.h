@@ -227,7 +227,10 @@ int tdse_set_hold(tdse_ctx *ctx, int32_t kind, int32_t slot, int32_t hold);
 /*
  * Advance every active simulation by up to n_steps steps (MeshSimulation.run,
  * sims.py:289-321). use_graph != 0 replays one captured hipGraph per step.
- * Finished simulations stay resident until removed.
+ * Finished simulations stay resident until removed. Inside one call, SH ADI
+ * simulations may carry a step's mask and the next step's explicit half into
+ * the previous replay; the call's last replay never does, so between calls
+ * every simulation is at an ordinary step boundary.
  */
 int tdse_run(tdse_ctx *ctx, int32_t n_steps, int32_t use_graph);
 
