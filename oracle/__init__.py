@@ -38,6 +38,11 @@ Pinning (what checks each part against something other than itself):
                             in atomic units; hydrogen R_nl values at the origin,
                             R_21 in closed form, orthogonality, normalisation.
   * time grid, data times — known answers of the int() truncation and cadence.
+  * A(t) quadrature       — simps_old against scipy.integrate.simpson on odd
+                            sample counts and the even='avg' composition on
+                            even ones (the rule of SciPy < 1.11, which the
+                            reference's Python 3.7 environment runs); exact on
+                            cubics.
   * eigenbasis            — ARPACK ``eigsh`` (the reference's routine) against
                             LAPACK ``eigh_tridiagonal``; hydrogen energies.
   * unit constants        — the derived atomic units (bohr, hartree, atomic
