@@ -277,8 +277,12 @@ __global__ void __launch_bounds__(MAXT) k_line_steps(LineSim* __restrict__ sims,
         if (line_is_data_step(n + 1, N, S.store_every))
             line_observe<M>(S, x, k, P, line_data_slot(n + 1, N, S.store_every), red);
     }
+    // the store addresses from an opaque copy of the base: not the load addresses kept live
+    // (in registers or spilled) across the whole step loop
+    double2* gs = g;
+    asm volatile("" : "+v"(gs));
 #pragma unroll
-    for (int i = 0; i < M; ++i) st2(g, (size_t)i * P, x[i]);
+    for (int i = 0; i < M; ++i) st2(gs, (size_t)i * P, x[i]);
     __syncthreads();
     if (k == 0) S.step = (n == N) ? N + 1 : n;
 }
