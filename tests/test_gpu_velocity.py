@@ -67,6 +67,19 @@ def test_velocity_gauge_every_rows_per_thread(device_ctx, rows):
     _check(_run(device_ctx, comp), case)
 
 
+@pytest.mark.parametrize("r_points", [4097, 6000, 8192])
+def test_velocity_gauge_radial_lines_past_4096_points(device_ctx, r_points):
+    """512-thread workgroups (P = 512, m = 10, 12, 16): the h2 kernel with one thread per chunk,
+    and the odd l-pairs' factors as launches of their own around single-line R-phases (two
+    lines of 512 chunks do not fit one workgroup)."""
+    case = sh_case(r_points=r_points, l_bound=5, t_half_as=12.0, n_basis_l=1, r_bound_bohr=r_points / 20,
+                   gauge="velocity")
+    comp = case["compiled"]
+    print(f"velocity SH long line r_points={r_points} P={comp.P} m={comp.m}", flush=True)
+    assert comp.P == 512 and comp.m >= 10
+    _check(_run(device_ctx, comp), case)
+
+
 def test_velocity_gauge_norm_by_l_and_sparse_data(device_ctx):
     case = sh_case(r_points=120, l_bound=10, t_half_as=60.0, store_every=7, norm_by_l=True, gauge="velocity")
     _check(_run(device_ctx, case["compiled"]), case, by_l=True)
