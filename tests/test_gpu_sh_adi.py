@@ -66,6 +66,18 @@ def test_sh_adi_every_rows_per_thread(device_ctx, rows):
     _check(_run(device_ctx, comp), case)
 
 
+@pytest.mark.parametrize("r_points", [4097, 6000, 8192])
+def test_sh_adi_radial_lines_past_4096_points(device_ctx, r_points):
+    """512-thread r-sweep workgroups (P = 512, m = 10, 12, 16), explicit and implicit with the
+    run-ahead, against the oracle."""
+    case = sh_case(r_points=r_points, l_bound=5, t_half_as=12.0, n_basis_l=1, r_bound_bohr=r_points / 20,
+                   method="adi")
+    comp = case["compiled"]
+    print(f"SH ADI long line r_points={r_points} P={comp.P} m={comp.m}", flush=True)
+    assert comp.P == 512 and comp.m >= 10
+    _check(_run(device_ctx, comp), case)
+
+
 def test_sh_adi_norm_by_l_sparse_data_and_direct_launches(device_ctx):
     case = sh_case(r_points=120, l_bound=10, t_half_as=60.0, store_every=7, norm_by_l=True, method="adi")
     graph = _run(device_ctx, case["compiled"])
