@@ -282,10 +282,16 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE, 
         cn_implicit_lowreg<M, decltype(coef), decltype(coef2), false, LANES, decltype(ratio)>(
             x, c, P, coef, coef2, lds, cmk(0.0, 0.0), cmk(0.0, 0.0), ln, ratio, ib);
         if (S.has_mask) {
+            // inside the inner radius (most points) the mask is 1: decided on r^2, without the root
+            const double in2 = S.m_inner * S.m_inner;
 #pragma unroll
             for (int i = 0; i < M; ++i) {
                 const int j = c * M + i;
-                const double mk = (line_live && j < nr) ? cyl_mask_r(S, radius(i)) : 0.0;
+                double mk = 0.0;
+                if (line_live && j < nr) {
+                    const double rho = rho_of(i);
+                    mk = (!KEEP && z * z + rho * rho < in2) ? 1.0 : cyl_mask_r(S, radius(i));
+                }
                 x[i] = cscale(mk, x[i]);
             }
         }
